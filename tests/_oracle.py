@@ -1,0 +1,181 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) -- test infrastructure.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TABLES = os.path.join(ROOT, "mantiis_amd", "data", "system_files")
+FIXTURES = os.path.join(ROOT, "tests", "golden", "reference_io")
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_L = C.POINTER(C.c_int64)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def load(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    path = os.path.join(ROOT, "oracle", name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+    lib = C.CDLL(path)
+    lib.orc_create.restype = _P
+    lib.orc_create.argtypes = [C.c_int, _I, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_char_p, C.c_int]
+    lib.orc_destroy.argtypes = [_P]
+    lib.orc_last_error.restype = C.c_char_p
+    lib.orc_globals.argtypes = [_P, _D]
+    lib.orc_globals_for.argtypes = [C.c_double, C.c_double, C.c_double, _D]
+    lib.orc_cell_count.restype = C.c_int64
+    lib.orc_cell_count.argtypes = [_P]
+    lib.orc_max_level.argtypes = [_P]
+    lib.orc_set_operators.argtypes = [_P, C.c_int, C.c_int]
+    lib.orc_set_interaction.argtypes = [_P, C.c_double, C.c_double]
+    lib.orc_grid.argtypes = [_P, _L, _I, _I, _L, _I, _I, _I]
+    lib.orc_reconstructed.argtypes = [_P, _L]
+    lib.orc_buffer.restype = C.c_int64
+    lib.orc_buffer.argtypes = [_P, C.c_int, C.c_int, _L]
+    lib.orc_links.restype = C.c_int64
+    lib.orc_links.argtypes = [_P, C.c_int, _L, _L, _I]
+    lib.orc_gp_cells.restype = C.c_int64
+    lib.orc_gp_cells.argtypes = [_P, _L]
+    lib.orc_get.restype = C.c_int64
+    lib.orc_get.argtypes = [_P, C.c_char_p, _D]
+    lib.orc_set.restype = C.c_int64
+    lib.orc_set.argtypes = [_P, C.c_char_p, _D]
+    lib.orc_method.argtypes = [_P, C.c_char_p, C.c_int]
+    lib.orc_run.argtypes = [_P, C.c_int, C.c_int, C.c_double]
+    lib.orc_steps_taken.argtypes = [_P]
+    lib.orc_diff.restype = C.c_double
+    lib.orc_diff.argtypes = [_P]
+    lib.orc_diff_history.restype = C.c_int64
+    lib.orc_diff_history.argtypes = [_P, _D]
+    lib.orc_complete_domain.argtypes = [_P, _D, _D, _D]
+    lib.orc_psi_negative.argtypes = [_P]
+    return lib
+
+
+def read_column_major(path, n, dtype=float):
+    """Shape::loadExistingModel (shape.hpp:253-285): value (i=row, j=col) = data[j*N+i]."""
+    data = np.loadtxt(path, dtype=float)
+    assert data.size == n * n, (path, data.size)
+    return np.ascontiguousarray(data.reshape(n, n).T).astype(dtype)
+
+
+def load_fixture_geometry(folder=FIXTURES, n=64):
+    pore = read_column_major(os.path.join(folder, "pore.dat"), n, np.int32)
+    kn = read_column_major(os.path.join(folder, "Kn.dat"), n)
+    lp = read_column_major(os.path.join(folder, "localporesize.dat"), n)
+    return pore, kn, lp
+
+
+class Oracle:
+    """One oracle simulation (one LB2D + Grid2D + Shape of the reference)."""
+
+    RUN_KIND = {"transport": 0, "adsorption": 1, "coupled": 2}
+
+    def __init__(self, pore, kn, locpore, dx=1e-9, Fbody=1e-8, T=300.0, P=2e6, multiblock=False,
+                 fast=False):
+        self.lib = load(fast)
+        self.n = int(pore.shape[0])
+        self._pore = np.ascontiguousarray(pore, dtype=np.int32)
+        self._kn = np.ascontiguousarray(kn, dtype=np.float64)
+        self._lp = np.ascontiguousarray(locpore, dtype=np.float64)
+        h = self.lib.orc_create(self.n, _ptr(self._pore, _I), _ptr(self._kn, _D), _ptr(self._lp, _D),
+                                dx, Fbody, T, P, TABLES.encode(), 1 if multiblock else 0)
+        if not h:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ncell(self):
+        return int(self.lib.orc_cell_count(self.h))
+
+    def globals(self):
+        out = np.zeros(10)
+        self.lib.orc_globals(self.h, _ptr(out, _D))
+        keys = ["rhoin", "mfp", "Cf", "T_lu", "a_alpha_lu", "ZG", "Crho", "Cu", "Cl", "cubic_ok"]
+        return dict(zip(keys, out.tolist()))
+
+    def set_operators(self, collision=2, wall=2):
+        self.lib.orc_set_operators(self.h, collision, wall)
+
+    def set_interaction(self, Gff=-1.0, Gfs=-12.7):
+        self.lib.orc_set_interaction(self.h, Gff, Gfs)
+
+    def get(self, name):
+        n = self.lib.orc_get(self.h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(n)
+        self.lib.orc_get(self.h, name.encode(), _ptr(out, _D))
+        return out
+
+    def set(self, name, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        n = self.lib.orc_set(self.h, name.encode(), _ptr(arr, _D))
+        if n < 0:
+            raise KeyError(name)
+
+    def method(self, name, lvl=1):
+        if self.lib.orc_method(self.h, name.encode(), lvl) != 0:
+            raise KeyError(name)
+
+    def run(self, kind, iters, tol=1e-10):
+        n = self.lib.orc_run(self.h, self.RUN_KIND[kind], iters, tol)
+        if n < 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+        return n
+
+    def diff_history(self):
+        n = self.lib.orc_diff_history(self.h, None)
+        out = np.zeros(n)
+        self.lib.orc_diff_history(self.h, _ptr(out, _D))
+        return out
+
+    def complete_domain(self):
+        N2 = self.n * self.n
+        ux, uy, rho = np.zeros(N2), np.zeros(N2), np.zeros(N2)
+        self.lib.orc_complete_domain(self.h, _ptr(ux, _D), _ptr(uy, _D), _ptr(rho, _D))
+        return ux, uy, rho
+
+    def grid(self):
+        n = self.ncell
+        gid = np.zeros(n, np.int64)
+        ij = np.zeros((n, 2), np.int32)
+        lvl = np.zeros(n, np.int32)
+        con = np.zeros((n, 9), np.int64)
+        bnd = np.zeros(n, np.int32)
+        isb = np.zeros(n, np.int32)
+        typ = np.zeros(n, np.int32)
+        self.lib.orc_grid(self.h, _ptr(gid, _L), _ptr(ij, _I), _ptr(lvl, _I), _ptr(con, _L), _ptr(bnd, _I),
+                          _ptr(isb, _I), _ptr(typ, _I))
+        return dict(gridID=gid, gridIJ=ij, level=lvl, connect=con, bnd=bnd, isBuffer=isb, type=typ)
+
+    def links(self, which):
+        w = {"stream": 0, "boundary": 1, "periodic": 2}[which]
+        n = self.lib.orc_links(self.h, w, None, None, None)
+        a = np.zeros(n, np.int64)
+        b = np.zeros(n, np.int64)
+        c = np.zeros(n, np.int32)
+        self.lib.orc_links(self.h, w, _ptr(a, _L), _ptr(b, _L), _ptr(c, _I))
+        return a, b, c
