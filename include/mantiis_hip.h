@@ -1,0 +1,210 @@
+/*
+ * mantiis_hip.h -- C ABI of the MI355X (gfx950) D2Q9 lattice-Boltzmann step.
+ *
+ * This is the drop-in boundary for the mantiis time step: the C++ classes
+ * LB2D / Grid2D / Shape / MantiisApp (the .hpp headers beside this file,
+ * named like include/{coreLB,grid,shape,mantiisApp}.hpp of the reference)
+ * are thin wrappers over these functions.  Plain pointers and sizes only; no
+ * exception crosses this boundary.  Every function returns MTS_OK (0) or a
+ * negative status; mts_last_error() gives the message of the calling thread's
+ * last failure.
+ *
+ * Threading: a handle is used from one host thread at a time; all work of a
+ * handle is issued on one HIP stream owned by the handle.  Functions that
+ * return host data synchronise that stream.
+ *
+ * Cell order: "compact" arrays follow the reference's cell list -- fluid
+ * pixels in row-major order for the single-block grid (grid.hpp:401-424,
+ * 427-889) -- so that a compact array here and an LB2D std::vector of the
+ * reference hold the same cell at the same index.
+ */
+#ifndef MANTIIS_HIP_H
+#define MANTIIS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference throws std::runtime_error / calls exit) */
+#define MTS_OK 0
+#define MTS_ERR_ARG (-1)         /* bad argument / size                                  */
+#define MTS_ERR_GEOMETRY (-2)    /* "[ERROR]: Boundary type is not contained." grid.hpp:1143-1150 */
+#define MTS_ERR_IO (-3)          /* missing / short input file, io.hpp:58-69             */
+#define MTS_ERR_HIP (-4)         /* HIP runtime failure                                  */
+#define MTS_ERR_OOM (-5)         /* device allocation failed                             */
+#define MTS_ERR_STATE (-6)       /* call not valid in the handle's current state         */
+#define MTS_ERR_UNSUPPORTED (-7) /* operator/precision combination not provided          */
+#define MTS_ERR_NONFINITE (-8)   /* NaN/Inf detected in a residual reduction             */
+#define MTS_ERR_PHYSICS (-9)     /* PR-EOS cubic has no single real root (utils.hpp:571-600) */
+
+/* ---- operator codes: Settings.collision_method / wall_boundary
+ * (mantiisApp.hpp:46-64, 180-234) */
+#define MTS_COLL_SRT 0
+#define MTS_COLL_MRT 1
+#define MTS_COLL_RMRT 2
+#define MTS_WALL_BB 0
+#define MTS_WALL_MDBB 1
+#define MTS_WALL_SRBB 2
+
+/* ---- storage precision of the fused step */
+#define MTS_FP64 0     /* populations and moments in FP64                         */
+#define MTS_FP32DEV 1  /* populations stored in FP32 as f_i - w_i*rho0; FP64 sums  */
+
+typedef struct mts_sim_s* mts_sim;
+
+/* Physical and numerical settings.  dx/Fbody/T/P/collision/wall mirror
+ * Settings (mantiisApp.hpp:46-64); Gff/Gfs are the Shan-Chen strengths that
+ * the reference fixes at compile time (globalParameters.hpp:120-121). */
+typedef struct mts_params {
+    double dx;       /* lattice spacing Cl [m]                 default 1e-9  */
+    double Fbody;    /* body force along +x, lattice units     default 1e-8  */
+    double T;        /* temperature [K]                        default 300   */
+    double P;        /* pressure [Pa]                          default 2e6   */
+    double Gff;      /* fluid-fluid strength                   default -1    */
+    double Gfs;      /* fluid-solid (adsorption) strength      default -12.7 */
+    int collision;   /* MTS_COLL_*                             default RMRT  */
+    int wall;        /* MTS_WALL_*                             default SRBB  */
+    int precision;   /* MTS_FP64 / MTS_FP32DEV                 default FP64  */
+    int multiblock;  /* 0 single-block grid (the only mode in this build)   */
+    int device;      /* HIP device ordinal                     default 0     */
+    int reserved;
+} mts_params;
+
+/* Fills the defaults above. */
+void mts_default_params(mts_params* p);
+
+/* Last error message of the calling thread ("" if none). */
+const char* mts_last_error(void);
+
+/* _GLOBAL_::recompute() (globalParameters.hpp:180-221): out[10] =
+ * {rhoin, mfp, Cf, T_lu, a_alpha_lu, Z_gas, Crho, Cu, Cl, cubic_ok}. */
+int mts_globals(const mts_params* p, double* out10);
+
+/* Shape::loadExistingModel (shape.hpp:253-285): reads folder/pore.dat,
+ * Kn.dat, localporesize.dat (column-major text, n*n values each) into
+ * row-major [y][x] arrays supplied by the caller. */
+int mts_load_existing_model(const char* folder, int n, int32_t* pore, double* kn, double* locpore);
+
+/* Grid2D(shape) + Grid2D::initialize + LB2D::LB2D (grid.hpp:266-311,
+ * coreLB.hpp:231-378): builds the cell list, neighbour connectivity,
+ * boundary types and link tables of a row-major [y][x] geometry
+ * (pore 1 = fluid), the per-cell relaxation times (coreLB.hpp:482-502) and
+ * the device state; then LB2D::initialize (coreLB.hpp:381-417). */
+int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
+               const mts_params* p, mts_sim* out);
+int mts_destroy(mts_sim s);
+
+/* Grid2D::globalGridSize (grid.hpp:195). */
+int mts_cell_count(mts_sim s, int64_t* n);
+
+/* Grid2D arrays in compact order (grid.hpp:170-186): gridID [n],
+ * gridIJ [n*2] = {x, y}, gridLevel [n], gridConnect [n*9], bndTypes [n].
+ * Any pointer may be NULL. */
+int mts_grid_info(mts_sim s, int64_t* gridID, int32_t* gridIJ, int32_t* level, int64_t* connect,
+                  int32_t* bnd);
+
+/* Link tables of LB2D::LB2D (coreLB.hpp:272-366) decoded from the device's
+ * per-cell link words: which = 0 stream (id, loc, ic), 1 boundary (id, ic),
+ * 2 periodic (id, i2, ic).  Call with NULL arrays to get *count first. */
+int mts_links(mts_sim s, int which, int64_t* id, int64_t* loc, int32_t* ic, int64_t* count);
+
+/* LB2D::setCollision / setWallBoundary (coreLB.hpp:794-807). */
+int mts_set_operators(mts_sim s, int collision, int wall);
+
+/* _GLOBAL_::Fbody (globalParameters.hpp:49), which MantiisApp::run sets to
+ * 0 for the adsorption warm-up of the coupled mode (mantiisApp.hpp:379-384). */
+int mts_set_body_force(mts_sim s, double Fbody);
+
+/* LB2D::initialize (coreLB.hpp:381-417): f = W*rho_in, then moments. */
+int mts_initialize(mts_sim s);
+
+/* One call = nsteps of MantiisApp::transport_simulation_loop
+ * (mantiisApp.hpp:239-246): equilibrium, evolutionStepOfMultiBlock,
+ * calculateMacroscopicPorperties, calculateVelocityDifference -- fused into
+ * one collide-stream pass per step.  The residual of the last step is
+ * available from mts_residual; f, rho, ux, uy, usq afterwards equal the
+ * reference's vectors after nsteps. */
+int mts_step_transport(mts_sim s, int nsteps);
+
+/* nsteps of MantiisApp::adsorption_simulation_loop (mantiisApp.hpp:248-265)
+ * (Shan-Chen psi, forces, MRT + Guo source, BB, periodic, generalized
+ * periodic, density residual), fused. */
+int mts_step_adsorption(mts_sim s, int nsteps);
+
+/* nsteps of MantiisApp::coupled_simulation_loop (mantiisApp.hpp:267-284). */
+int mts_step_coupled(mts_sim s, int nsteps);
+
+/* LB2D::diff (coreLB.hpp:154) after the last step call. */
+int mts_residual(mts_sim s, double* diff);
+
+/* Unfused LB2D methods, one HIP kernel each, for phase-level use by the
+ * LB2D wrapper.  lvl is the level argument of the reference method. */
+#define MTS_M_EQUILIBRIUM 1            /* coreLB.hpp:420-430   */
+#define MTS_M_REGULARIZE 2             /* 646-675              */
+#define MTS_M_SRT 3                    /* 678-691              */
+#define MTS_M_MRT 4                    /* 694-742              */
+#define MTS_M_RMRT 5                   /* 744-792              */
+#define MTS_M_COLLIDE 6                /* 809-812              */
+#define MTS_M_ZERODOWN 7               /* 1233-1246            */
+#define MTS_M_STREAM 8                 /* 834-856              */
+#define MTS_M_BBWALL 9                 /* 859-865              */
+#define MTS_M_SRBBWALL 10              /* 918-933              */
+#define MTS_M_MDBBWALL 11              /* 936-967              */
+#define MTS_M_PERIODIC 12              /* 969-977              */
+#define MTS_M_GENPERIODIC 13           /* 980-1009             */
+#define MTS_M_MACROSCOPIC 14           /* 1018-1054            */
+#define MTS_M_DENSITY 15               /* 1056-1067            */
+#define MTS_M_VELOCITY 16              /* 1069-1099            */
+#define MTS_M_PSEUDOPOTENTIAL 17       /* 515-531              */
+#define MTS_M_FORCES 18                /* 533-566              */
+#define MTS_M_SOURCE 19                /* 568-630              */
+#define MTS_M_VELOCITY_DIFF 20         /* 1248-1267            */
+#define MTS_M_DENSITY_DIFF 21          /* 1269-1283            */
+#define MTS_M_APPLY_BC 22              /* 814-831              */
+#define MTS_M_EVOLUTION 23             /* 1310-1331            */
+#define MTS_M_COPY_F_FTEMP 24          /* ftemp = f            */
+#define MTS_M_COPY_F_FB 25             /* fb = f               */
+#define MTS_M_COPY_USQ_OLD 26          /* usq_old = usq        */
+#define MTS_M_COPY_RHO_OLD 27          /* rho_old = rho        */
+int mts_method(mts_sim s, int method, int lvl);
+
+/* Public vectors of LB2D (coreLB.hpp:89-114) in compact order.
+ * Population fields are AoS [cell*9 + dir] like the reference. */
+#define MTS_F_F 0
+#define MTS_F_FTEMP 1
+#define MTS_F_FEQ 2
+#define MTS_F_FNEQ 3
+#define MTS_F_FB 4
+#define MTS_F_RHO 5
+#define MTS_F_RHO_OLD 6
+#define MTS_F_UX 7
+#define MTS_F_UY 8
+#define MTS_F_USQ 9
+#define MTS_F_USQ_OLD 10
+#define MTS_F_PSI 11
+#define MTS_F_FX 12
+#define MTS_F_FY 13
+#define MTS_F_TAUS 14
+#define MTS_F_KN 15
+int mts_field_size(mts_sim s, int field, int64_t* n);
+int mts_get_field(mts_sim s, int field, double* out);
+int mts_set_field(mts_sim s, int field, const double* in);
+
+/* LB2D::completeSingleGridDomain (coreLB.hpp:1344-1411): ux, uy, rho as
+ * nx*ny row-major arrays with solid pixels set to 0. */
+int mts_complete_domain(mts_sim s, double* ux, double* uy, double* rho);
+
+/* FP64 diagnostics of the current state (no reference counterpart):
+ * out4 = {sum rho, sum j_x, sum j_y, fluid cell count}. */
+int mts_diagnostics(mts_sim s, double* out4);
+
+/* The HIP stream (hipStream_t) on which the handle launches its kernels. */
+int mts_get_stream(mts_sim s, void** stream);
+int mts_synchronize(mts_sim s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MANTIIS_HIP_H */

@@ -1,0 +1,611 @@
+// C ABI of the library (include/mantiis_hip.h): handle life cycle, device
+// memory, pass sequencing of the fused steps, per-method entry points and
+// host/device transfers.  No exception crosses the boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "mantiis_hip.h"
+#include "mts_common.h"
+#include "mts_kernels.h"
+
+namespace mts {
+void upload_xynorm(hipStream_t st);
+}
+
+using namespace mts;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? MTS_ERR_OOM : MTS_ERR_HIP,               \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+struct mts_sim_s {
+    Geometry geo;
+    Physics ph;
+    mts_params prm{};
+    hipStream_t stream = nullptr;
+    int64_t P = 0;
+    uint32_t* d_meta = nullptr;
+    uint16_t* d_pidx = nullptr;
+    CellParam* d_params = nullptr;
+    int32_t* d_cellPos = nullptr;
+    void* d_A = nullptr;
+    void* d_B = nullptr;
+    double* d_f = nullptr;
+    double *d_rho = nullptr, *d_rho_old = nullptr, *d_ux = nullptr, *d_uy = nullptr, *d_usq = nullptr,
+           *d_usq_old = nullptr;
+    double *d_ftemp = nullptr, *d_feq = nullptr, *d_fneq = nullptr, *d_fb = nullptr, *d_psi = nullptr,
+           *d_Fx = nullptr, *d_Fy = nullptr;
+    double* d_part = nullptr;
+    double* d_red = nullptr;
+    double* d_compact = nullptr;
+    int* d_neg = nullptr;
+    int nparts = 0;
+    double diff = 1.0;
+    double Fbody = 1e-8;
+    std::vector<void*> allocs;
+
+    LaunchCtx ctx() const {
+        LaunchCtx c;
+        c.stream = stream;
+        c.nx = geo.nx;
+        c.ny = geo.ny;
+        c.stride = geo.stride;
+        c.ncell = geo.ncell;
+        c.meta = d_meta;
+        c.pidx = d_pidx;
+        c.params = d_params;
+        c.cellPos = d_cellPos;
+        c.fp32 = prm.precision == MTS_FP32DEV;
+        c.coll = prm.collision;
+        c.wall = prm.wall;
+        c.rho0 = ph.rhoin;
+        c.rhoin = ph.rhoin;
+        c.Fbody = Fbody;
+        c.Gff = prm.Gff;
+        c.Gfs = prm.Gfs;
+        c.T_lu = ph.T_lu;
+        c.a_alpha_lu = ph.a_alpha_lu;
+        c.gpIn = geo.gpIn;
+        c.gpOut = geo.gpOut;
+        return c;
+    }
+    template <class T>
+    int alloc(T** p, size_t n) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        e = hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(T), stream);
+        if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return MTS_OK;
+    }
+    ~mts_sim_s() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* q : allocs) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int ensure_phase() {
+        if (d_ftemp) return MTS_OK;
+        const size_t np = (size_t)geo.stride * 9;
+        int rc;
+        if ((rc = alloc(&d_ftemp, np)) || (rc = alloc(&d_feq, np)) || (rc = alloc(&d_fneq, np)) ||
+            (rc = alloc(&d_fb, np)))
+            return rc;
+        return MTS_OK;
+    }
+    int ensure_sc() {
+        if (d_psi) return MTS_OK;
+        int rc;
+        if ((rc = alloc(&d_psi, P)) || (rc = alloc(&d_Fx, P)) || (rc = alloc(&d_Fy, P))) return rc;
+        return MTS_OK;
+    }
+    int launched() {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        return MTS_OK;
+    }
+    int reduce(int kind, const double* a0, const double* a1, const double* a2, double* out4) {
+        launch_reduce(ctx(), kind, a0, a1, a2, d_part, nparts, d_red);
+        int rc = launched();
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(out4, d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return MTS_OK;
+    }
+};
+
+extern "C" {
+
+void mts_default_params(mts_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->dx = 1.0e-9;
+    p->Fbody = 1e-8;
+    p->T = 300.0;
+    p->P = 2.0e6;
+    p->Gff = -1.0;
+    p->Gfs = -12.7;
+    p->collision = MTS_COLL_RMRT;
+    p->wall = MTS_WALL_SRBB;
+    p->precision = MTS_FP64;
+    p->multiblock = 0;
+    p->device = 0;
+}
+
+const char* mts_last_error(void) { return g_err.c_str(); }
+
+int mts_globals(const mts_params* p, double* out) {
+    if (!p || !out) return fail(MTS_ERR_ARG, "null argument");
+    Physics ph;
+    ph.Cl = p->dx;
+    ph.Fbody = p->Fbody;
+    ph.T_phy = p->T;
+    ph.P_phy = p->P;
+    compute_physics(ph);
+    const double v[10] = {ph.rhoin, ph.mfp, ph.Cf, ph.T_lu, ph.a_alpha_lu, ph.Z, ph.Crho, ph.Cu, ph.Cl, ph.cubic_ok ? 1.0 : 0.0};
+    std::memcpy(out, v, sizeof(v));
+    return ph.cubic_ok ? MTS_OK : fail(MTS_ERR_PHYSICS, "PR-EOS: no single positive real root at this (T, P)");
+}
+
+int mts_load_existing_model(const char* folder, int n, int32_t* pore, double* kn, double* locpore) {
+    if (!folder || n <= 0 || !pore || !kn || !locpore) return fail(MTS_ERR_ARG, "null argument");
+    const char* names[3] = {"pore.dat", "Kn.dat", "localporesize.dat"};
+    std::vector<double> data((size_t)n * n);
+    for (int k = 0; k < 3; k++) {
+        std::string fn = std::string(folder) + names[k];
+        std::ifstream in(fn);
+        if (!in.is_open()) return fail(MTS_ERR_IO, "[ERROR]: File not found. (" + fn + ")");
+        for (size_t i = 0; i < data.size(); i++)
+            if (!(in >> data[i])) return fail(MTS_ERR_IO, "[ERROR]: Insufficient data in the file. (" + fn + ")");
+        // column-major on disk: value (row i, col j) = data[j*n + i]
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                const double v = data[(size_t)j * n + i];
+                if (k == 0) pore[(size_t)i * n + j] = (int32_t)v;
+                else if (k == 1) kn[(size_t)i * n + j] = v;
+                else locpore[(size_t)i * n + j] = v;
+            }
+    }
+    return MTS_OK;
+}
+
+int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore, const mts_params* p,
+               mts_sim* out) {
+    if (!out) return fail(MTS_ERR_ARG, "null handle pointer");
+    *out = nullptr;
+    if (!pore || !kn || !locpore) return fail(MTS_ERR_ARG, "null geometry array");
+    mts_params prm;
+    if (p) prm = *p;
+    else mts_default_params(&prm);
+    if (prm.multiblock) return fail(MTS_ERR_UNSUPPORTED, "multi-block grids are not provided by this build");
+    if (prm.precision != MTS_FP64 && prm.precision != MTS_FP32DEV) return fail(MTS_ERR_ARG, "bad precision");
+    if (prm.collision < 0 || prm.collision > 2) return fail(MTS_ERR_ARG, "Incorrect keyword for collision");
+    if (prm.wall < 0 || prm.wall > 2) return fail(MTS_ERR_ARG, "Incorrect keyword for wall boundary method");
+    if ((int64_t)nx * ny > (int64_t)1 << 31) return fail(MTS_ERR_ARG, "lattice larger than 2^31 sites");
+    mts_sim_s* s = new (std::nothrow) mts_sim_s();
+    if (!s) return fail(MTS_ERR_OOM, "host allocation");
+    s->prm = prm;
+    s->Fbody = prm.Fbody;
+    s->ph.Cl = prm.dx;
+    s->ph.Fbody = prm.Fbody;
+    s->ph.T_phy = prm.T;
+    s->ph.P_phy = prm.P;
+    compute_physics(s->ph);
+    if (!s->ph.cubic_ok) {
+        delete s;
+        return fail(MTS_ERR_PHYSICS, "PR-EOS: no single positive real root at this (T, P)");
+    }
+    std::string err;
+    int rc = build_single_grid(nx, ny, pore, kn, locpore, s->ph, s->geo, err);
+    if (rc) {
+        delete s;
+        return fail(rc, err);
+    }
+    auto bail = [&](int code) {
+        delete s;
+        return code;
+    };
+    hipError_t e = hipSetDevice(prm.device);
+    if (e != hipSuccess) return bail(fail(MTS_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
+    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(fail(MTS_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+    const Geometry& g = s->geo;
+    s->P = (int64_t)g.nx * g.ny;
+    const size_t np = (size_t)g.stride * 9;
+    const size_t esz = prm.precision == MTS_FP32DEV ? 4 : 8;
+    s->nparts = (int)std::min<int64_t>(1024, (s->P + 255) / 256);
+    if ((rc = s->alloc(&s->d_meta, s->P)) || (rc = s->alloc(&s->d_pidx, s->P)) ||
+        (rc = s->alloc(&s->d_params, g.params.size())) || (rc = s->alloc(&s->d_cellPos, g.ncell)) ||
+        (rc = s->alloc((char**)&s->d_A, np * esz)) || (rc = s->alloc((char**)&s->d_B, np * esz)) ||
+        (rc = s->alloc(&s->d_f, np)) || (rc = s->alloc(&s->d_rho, s->P)) || (rc = s->alloc(&s->d_rho_old, s->P)) ||
+        (rc = s->alloc(&s->d_ux, s->P)) || (rc = s->alloc(&s->d_uy, s->P)) || (rc = s->alloc(&s->d_usq, s->P)) ||
+        (rc = s->alloc(&s->d_usq_old, s->P)) || (rc = s->alloc(&s->d_part, (size_t)4 * s->nparts)) ||
+        (rc = s->alloc(&s->d_red, 4)) || (rc = s->alloc(&s->d_neg, 1)))
+        return bail(rc);
+    e = hipMemcpyAsync(s->d_meta, g.meta.data(), s->P * 4, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_pidx, g.pidx.data(), s->P * 2, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_params, g.params.data(), g.params.size() * sizeof(CellParam), hipMemcpyHostToDevice,
+                           s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_cellPos, g.cellPos.data(), g.ncell * 4, hipMemcpyHostToDevice, s->stream);
+    if (e != hipSuccess) return bail(fail(MTS_ERR_HIP, std::string("upload: ") + hipGetErrorString(e)));
+    upload_icsr(s->stream);
+    upload_xynorm(s->stream);
+    *out = s;
+    rc = mts_initialize(s);
+    if (rc) {
+        *out = nullptr;
+        return bail(rc);
+    }
+    return MTS_OK;
+}
+
+int mts_destroy(mts_sim s) {
+    delete s;
+    return MTS_OK;
+}
+
+int mts_cell_count(mts_sim s, int64_t* n) {
+    if (!s || !n) return fail(MTS_ERR_ARG, "null argument");
+    *n = s->geo.ncell;
+    return MTS_OK;
+}
+
+int mts_grid_info(mts_sim s, int64_t* gridID, int32_t* gridIJ, int32_t* level, int64_t* connect, int32_t* bnd) {
+    if (!s) return fail(MTS_ERR_ARG, "null handle");
+    const Geometry& g = s->geo;
+    for (int64_t c = 0; c < g.ncell; c++) {
+        const int64_t p = g.cellPos[c];
+        if (gridID) gridID[c] = p;  // sub2ind({ny,nx,1},{x,y,0}) == y*nx + x for square lattices
+        if (gridIJ) {
+            gridIJ[2 * c] = (int32_t)(p % g.nx);
+            gridIJ[2 * c + 1] = (int32_t)(p / g.nx);
+        }
+        if (level) level[c] = 1;
+        if (bnd) bnd[c] = g.bndTypes[c];
+        if (connect)
+            for (int k = 0; k < 9; k++) connect[9 * c + k] = g.gridConnect[9 * c + k];
+    }
+    return MTS_OK;
+}
+
+int mts_links(mts_sim s, int which, int64_t* id, int64_t* loc, int32_t* ic, int64_t* count) {
+    if (!s || !count || which < 0 || which > 2) return fail(MTS_ERR_ARG, "bad argument");
+    Geometry g = s->geo;  // decode from the words held on the device
+    HIP_TRY(hipMemcpyAsync(g.meta.data(), s->d_meta, s->P * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<int64_t> a, b;
+    std::vector<int32_t> c;
+    decode_links(g, which, a, b, c);
+    *count = (int64_t)a.size();
+    if (id) std::copy(a.begin(), a.end(), id);
+    if (loc) std::copy(b.begin(), b.end(), loc);
+    if (ic) std::copy(c.begin(), c.end(), ic);
+    return MTS_OK;
+}
+
+int mts_set_operators(mts_sim s, int collision, int wall) {
+    if (!s) return fail(MTS_ERR_ARG, "null handle");
+    if (collision < 0 || collision > 2)
+        return fail(MTS_ERR_ARG,
+                    "Incorrect keyword for collision: available options are\n\t 2 for regularized MRT\n\t 1 for simple MRT\n\t 0 for simple BGK\n");
+    if (wall < 0 || wall > 2)
+        return fail(MTS_ERR_ARG,
+                    "Incorrect keyword for wall boundary method: available options are\n\t 2 for SRBBWall\n\t 1 for MDBBWall\n\t 0 for BBWall\n");
+    s->prm.collision = collision;
+    s->prm.wall = wall;
+    return MTS_OK;
+}
+
+int mts_set_body_force(mts_sim s, double Fbody) {
+    if (!s) return fail(MTS_ERR_ARG, "null handle");
+    s->Fbody = Fbody;
+    return MTS_OK;
+}
+
+int mts_initialize(mts_sim s) {
+    if (!s) return fail(MTS_ERR_ARG, "null handle");
+    LaunchCtx c = s->ctx();
+    launch_init(c, s->d_f);
+    launch_macro(c, s->d_f, s->d_rho, s->d_ux, s->d_uy, s->d_usq);
+    int rc = s->launched();
+    if (rc) return rc;
+    s->diff = 1.0;
+    return MTS_OK;
+}
+
+// ---- unfused per-step fallback (operators without a fused kernel: MDBB)
+static int transport_step_phases(mts_sim s) {
+    int rc;
+    if ((rc = mts_method(s, MTS_M_COPY_USQ_OLD, 1)) || (rc = mts_method(s, MTS_M_EQUILIBRIUM, 1)) ||
+        (rc = mts_method(s, MTS_M_EVOLUTION, 1)) || (rc = mts_method(s, MTS_M_MACROSCOPIC, 1)) ||
+        (rc = mts_method(s, MTS_M_VELOCITY_DIFF, 1)))
+        return rc;
+    return MTS_OK;
+}
+
+int mts_step_transport(mts_sim s, int nsteps) {
+    if (!s) return fail(MTS_ERR_ARG, "null handle");
+    if (nsteps <= 0) return fail(MTS_ERR_ARG, "nsteps must be positive");
+    if (s->prm.wall == MTS_WALL_MDBB) {
+        if (s->prm.precision != MTS_FP64) return fail(MTS_ERR_UNSUPPORTED, "MDBB wall is provided in FP64 only");
+        for (int i = 0; i < nsteps; i++) {
+            int rc = transport_step_phases(s);
+            if (rc) return rc;
+        }
+        return MTS_OK;
+    }
+    LaunchCtx c = s->ctx();
+    TransportSpec t;
+    int rc;
+    // pass 0: collide the canonical state
+    t.fcan = s->d_f;
+    t.Aout = s->d_A;
+    t.usq_old = nsteps == 1 ? s->d_usq_old : nullptr;
+    if ((rc = launch_transport(c, 0, t))) return fail(rc, "operator combination has no fused kernel");
+    for (int i = 1; i < nsteps; i++) {
+        TransportSpec u;
+        u.Ain = s->d_A;
+        u.Aout = s->d_B;
+        u.usq_old = (i == nsteps - 1) ? s->d_usq_old : nullptr;
+        launch_transport(c, 1, u);
+        std::swap(s->d_A, s->d_B);
+    }
+    TransportSpec z;
+    z.Ain = s->d_A;
+    z.fcan = s->d_f;
+    z.rho = s->d_rho;
+    z.ux = s->d_ux;
+    z.uy = s->d_uy;
+    z.usq = s->d_usq;
+    launch_transport(c, 2, z);
+    if ((rc = s->launched())) return rc;
+    double r[4];
+    if ((rc = s->reduce(0, s->d_usq, s->d_usq_old, nullptr, r))) return rc;
+    s->diff = r[0] / r[1];
+    return MTS_OK;
+}
+
+static int sc_steps(mts_sim s, int kind, int nsteps) {
+    if (!s) return fail(MTS_ERR_ARG, "null handle");
+    if (nsteps <= 0) return fail(MTS_ERR_ARG, "nsteps must be positive");
+    int rc;
+    if ((rc = s->ensure_sc())) return rc;
+    LaunchCtx c = s->ctx();
+    HIP_TRY(hipMemsetAsync(s->d_neg, 0, sizeof(int), s->stream));
+    if (kind == 0 && nsteps == 1)  // rho_old = rho
+        HIP_TRY(hipMemcpyAsync(s->d_rho_old, s->d_rho, s->P * 8, hipMemcpyDeviceToDevice, s->stream));
+    for (int i = 0; i < nsteps; i++) {
+        ScSpec sp;
+        sp.fcan = s->d_f;
+        sp.Ain = s->d_A;
+        sp.Aout = i == 0 ? s->d_A : s->d_B;
+        if (i == nsteps - 1) {
+            sp.rho = s->d_rho;
+            sp.ux = s->d_ux;
+            sp.uy = s->d_uy;
+            sp.usq = s->d_usq;
+            sp.psi = s->d_psi;
+            sp.Fx = s->d_Fx;
+            sp.Fy = s->d_Fy;
+            sp.psi_negative = s->d_neg;
+        } else if (i == nsteps - 2 && kind == 0) {
+            // the density of the step before the last is the residual's rho_old
+            sp.rho = s->d_rho_old;
+            sp.ux = s->d_ux;
+            sp.uy = s->d_uy;
+            sp.usq = s->d_usq;
+        }
+        launch_shanchen(c, kind, i == 0 ? 0 : 1, sp);
+        if (i > 0) std::swap(s->d_A, s->d_B);
+    }
+    ScSpec z;
+    z.Ain = s->d_A;
+    z.fcan = s->d_f;
+    launch_shanchen(c, kind, 2, z);
+    if ((rc = s->launched())) return rc;
+    double r[4];
+    if (kind == 0) {
+        if ((rc = s->reduce(1, s->d_rho, s->d_rho_old, nullptr, r))) return rc;
+        double d = r[0];
+        if (d < 1.0e-15) d = 1.0e-5;  // coreLB.hpp:1280
+        s->diff = d;
+    } else {
+        // coreLB.hpp:1248-1267 with usq_old never refreshed by this loop
+        if ((rc = s->reduce(0, s->d_usq, s->d_usq_old, nullptr, r))) return rc;
+        s->diff = r[0] / r[1];
+    }
+    if (!std::isfinite(s->diff)) return fail(MTS_ERR_NONFINITE, "residual is not finite");
+    return MTS_OK;
+}
+
+int mts_step_adsorption(mts_sim s, int nsteps) { return sc_steps(s, 0, nsteps); }
+int mts_step_coupled(mts_sim s, int nsteps) { return sc_steps(s, 1, nsteps); }
+
+int mts_residual(mts_sim s, double* diff) {
+    if (!s || !diff) return fail(MTS_ERR_ARG, "null argument");
+    *diff = s->diff;
+    return MTS_OK;
+}
+
+int mts_method(mts_sim s, int method, int lvl) {
+    if (!s) return fail(MTS_ERR_ARG, "null handle");
+    int rc;
+    if ((rc = s->ensure_phase()) || (rc = s->ensure_sc())) return rc;
+    const size_t np = (size_t)s->geo.stride * 9 * 8;
+    switch (method) {
+        case MTS_M_COPY_F_FTEMP:
+            HIP_TRY(hipMemcpyAsync(s->d_ftemp, s->d_f, np, hipMemcpyDeviceToDevice, s->stream));
+            return MTS_OK;
+        case MTS_M_COPY_F_FB:
+            HIP_TRY(hipMemcpyAsync(s->d_fb, s->d_f, np, hipMemcpyDeviceToDevice, s->stream));
+            return MTS_OK;
+        case MTS_M_COPY_USQ_OLD:
+            HIP_TRY(hipMemcpyAsync(s->d_usq_old, s->d_usq, s->P * 8, hipMemcpyDeviceToDevice, s->stream));
+            return MTS_OK;
+        case MTS_M_COPY_RHO_OLD:
+            HIP_TRY(hipMemcpyAsync(s->d_rho_old, s->d_rho, s->P * 8, hipMemcpyDeviceToDevice, s->stream));
+            return MTS_OK;
+        case MTS_M_VELOCITY_DIFF: {
+            double r[4];
+            if ((rc = s->reduce(0, s->d_usq, s->d_usq_old, nullptr, r))) return rc;
+            s->diff = r[0] / r[1];
+            return MTS_OK;
+        }
+        case MTS_M_DENSITY_DIFF: {
+            double r[4];
+            if ((rc = s->reduce(1, s->d_rho, s->d_rho_old, nullptr, r))) return rc;
+            s->diff = r[0] < 1.0e-15 ? 1.0e-5 : r[0];
+            return MTS_OK;
+        }
+        case MTS_M_EVOLUTION: {  // coreLB.hpp:1310-1331 at one level
+            if (lvl != 1) return MTS_OK;
+            if ((rc = mts_method(s, MTS_M_COPY_F_FB, 1)) || (rc = mts_method(s, MTS_M_COLLIDE, 1)) ||
+                (rc = mts_method(s, MTS_M_COPY_F_FTEMP, 1)) || (rc = mts_method(s, MTS_M_ZERODOWN, 1)) ||
+                (rc = mts_method(s, MTS_M_STREAM, 1)) || (rc = mts_method(s, MTS_M_APPLY_BC, 1)))
+                return rc;
+            return MTS_OK;
+        }
+        default: break;
+    }
+    PhaseArrays a{s->d_f, s->d_ftemp, s->d_feq, s->d_fneq, s->d_fb, s->d_rho, s->d_rho_old, s->d_ux, s->d_uy,
+                  s->d_usq, s->d_usq_old, s->d_psi, s->d_Fx, s->d_Fy};
+    rc = launch_phase(s->ctx(), method, lvl, a, s->d_neg);
+    if (rc) return fail(rc, "unknown method id");
+    return s->launched();
+}
+
+static double* field_ptr(mts_sim s, int field, int* q) {
+    *q = 1;
+    switch (field) {
+        case MTS_F_F: *q = 9; return s->d_f;
+        case MTS_F_FTEMP: *q = 9; return s->d_ftemp;
+        case MTS_F_FEQ: *q = 9; return s->d_feq;
+        case MTS_F_FNEQ: *q = 9; return s->d_fneq;
+        case MTS_F_FB: *q = 9; return s->d_fb;
+        case MTS_F_RHO: return s->d_rho;
+        case MTS_F_RHO_OLD: return s->d_rho_old;
+        case MTS_F_UX: return s->d_ux;
+        case MTS_F_UY: return s->d_uy;
+        case MTS_F_USQ: return s->d_usq;
+        case MTS_F_USQ_OLD: return s->d_usq_old;
+        case MTS_F_PSI: return s->d_psi;
+        case MTS_F_FX: return s->d_Fx;
+        case MTS_F_FY: return s->d_Fy;
+        default: return nullptr;
+    }
+}
+
+int mts_field_size(mts_sim s, int field, int64_t* n) {
+    if (!s || !n) return fail(MTS_ERR_ARG, "null argument");
+    if (field < 0 || field > MTS_F_KN) return fail(MTS_ERR_ARG, "unknown field");
+    const bool pop = field <= MTS_F_FB;
+    *n = s->geo.ncell * (pop ? 9 : 1);
+    return MTS_OK;
+}
+
+int mts_get_field(mts_sim s, int field, double* out) {
+    if (!s || !out) return fail(MTS_ERR_ARG, "null argument");
+    if (field == MTS_F_TAUS || field == MTS_F_KN) {
+        const auto& v = field == MTS_F_TAUS ? s->geo.taus : s->geo.kn;
+        std::copy(v.begin(), v.end(), out);
+        return MTS_OK;
+    }
+    int rc;
+    if (field == MTS_F_FTEMP || field == MTS_F_FEQ || field == MTS_F_FNEQ || field == MTS_F_FB)
+        if ((rc = s->ensure_phase())) return rc;
+    if (field == MTS_F_PSI || field == MTS_F_FX || field == MTS_F_FY)
+        if ((rc = s->ensure_sc())) return rc;
+    int q;
+    double* d = field_ptr(s, field, &q);
+    if (!d) return fail(MTS_ERR_ARG, "unknown field");
+    const int64_t n = s->geo.ncell * q;
+    double* tmp = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&tmp, n * 8, s->stream));
+    LaunchCtx c = s->ctx();
+    const int64_t stride = q == 9 ? s->geo.stride : 0;
+    LaunchCtx c2 = c;
+    c2.stride = stride;
+    launch_gather(c2, d, q, tmp);
+    if ((rc = s->launched())) return rc;
+    HIP_TRY(hipMemcpyAsync(out, tmp, n * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipFreeAsync(tmp, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MTS_OK;
+}
+
+int mts_set_field(mts_sim s, int field, const double* in) {
+    if (!s || !in) return fail(MTS_ERR_ARG, "null argument");
+    if (field == MTS_F_TAUS || field == MTS_F_KN) return fail(MTS_ERR_ARG, "read-only field");
+    int rc;
+    if (field == MTS_F_FTEMP || field == MTS_F_FEQ || field == MTS_F_FNEQ || field == MTS_F_FB)
+        if ((rc = s->ensure_phase())) return rc;
+    if (field == MTS_F_PSI || field == MTS_F_FX || field == MTS_F_FY)
+        if ((rc = s->ensure_sc())) return rc;
+    int q;
+    double* d = field_ptr(s, field, &q);
+    if (!d) return fail(MTS_ERR_ARG, "unknown field");
+    const int64_t n = s->geo.ncell * q;
+    double* tmp = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&tmp, n * 8, s->stream));
+    HIP_TRY(hipMemcpyAsync(tmp, in, n * 8, hipMemcpyHostToDevice, s->stream));
+    LaunchCtx c2 = s->ctx();
+    c2.stride = q == 9 ? s->geo.stride : 0;
+    launch_scatter(c2, d, q, tmp);
+    if ((rc = s->launched())) return rc;
+    HIP_TRY(hipFreeAsync(tmp, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MTS_OK;
+}
+
+int mts_complete_domain(mts_sim s, double* ux, double* uy, double* rho) {
+    if (!s) return fail(MTS_ERR_ARG, "null handle");
+    double* outs[3] = {ux, uy, rho};
+    double* src[3] = {s->d_ux, s->d_uy, s->d_rho};
+    for (int k = 0; k < 3; k++) {
+        if (!outs[k]) continue;
+        // dense site image is row-major and holds 0 at solid sites
+        HIP_TRY(hipMemcpyAsync(outs[k], src[k], s->P * 8, hipMemcpyDeviceToHost, s->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MTS_OK;
+}
+
+int mts_diagnostics(mts_sim s, double* out4) {
+    if (!s || !out4) return fail(MTS_ERR_ARG, "null argument");
+    return s->reduce(2, s->d_rho, s->d_ux, s->d_uy, out4);
+}
+
+int mts_get_stream(mts_sim s, void** stream) {
+    if (!s || !stream) return fail(MTS_ERR_ARG, "null argument");
+    *stream = (void*)s->stream;
+    return MTS_OK;
+}
+
+int mts_synchronize(mts_sim s) {
+    if (!s) return fail(MTS_ERR_ARG, "null handle");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MTS_OK;
+}
+
+}  // extern "C"

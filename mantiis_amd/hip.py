@@ -1,0 +1,245 @@
+"""ctypes binding of libmantiis_hip.so (the C ABI of include/mantiis_hip.h).
+
+There is no fallback: if the library is missing or fails to load, every entry
+point raises HipLibraryMissing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmantiis_hip.so")
+
+COLL = {"SRT": 0, "MRT": 1, "RMRT": 2}
+WALL = {"BB": 0, "MDBB": 1, "SRBB": 2}
+PREC = {"fp64": 0, "fp32dev": 1}
+METHODS = {
+    "equilibrium": 1, "regularize": 2, "SRTCollision": 3, "MRTCollisionNonRegularized": 4,
+    "MRTCollisionRegularized": 5, "collide": 6, "zeroDown": 7, "stream": 8, "BBWall": 9, "SRBBWall": 10,
+    "MDBBWall": 11, "periodicBoundary": 12, "generalizedPeriodicBoundaryAdsorption": 13,
+    "calculateMacroscopicPorperties": 14, "calculateDensity": 15, "calculateVelocity": 16,
+    "getPseudoPotential": 17, "calculateForces": 18, "applySourceTerm": 19,
+    "calculateVelocityDifference": 20, "calculateDensityDifference": 21, "applyBoundaryConditions": 22,
+    "evolutionStepOfMultiBlock": 23, "copy_f_to_ftemp": 24, "copy_f_to_fb": 25, "copy_usq_to_usq_old": 26,
+    "copy_rho_to_rho_old": 27,
+}
+FIELDS = {"f": 0, "ftemp": 1, "feq": 2, "fneq": 3, "fb": 4, "rho": 5, "rho_old": 6, "ux": 7, "uy": 8,
+          "usq": 9, "usq_old": 10, "psi": 11, "Fx": 12, "Fy": 13, "taus": 14, "kn": 15}
+SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_existing_model", "mts_create",
+           "mts_destroy", "mts_cell_count", "mts_grid_info", "mts_links", "mts_set_operators",
+           "mts_set_body_force", "mts_initialize", "mts_step_transport", "mts_step_adsorption",
+           "mts_step_coupled", "mts_residual", "mts_method", "mts_field_size", "mts_get_field",
+           "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_get_stream", "mts_synchronize"]
+
+
+class HipLibraryMissing(RuntimeError):
+    pass
+
+
+class MtsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mts error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("dx", C.c_double), ("Fbody", C.c_double), ("T", C.c_double), ("P", C.c_double),
+                ("Gff", C.c_double), ("Gfs", C.c_double), ("collision", C.c_int), ("wall", C.c_int),
+                ("precision", C.c_int), ("multiblock", C.c_int), ("device", C.c_int), ("reserved", C.c_int)]
+
+
+_LIB = None
+_D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+
+
+def load_library(path=LIB_PATH):
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise HipLibraryMissing(f"{path} is not built (run __graft_entry__.build())")
+    try:
+        lib = C.CDLL(path)
+    except OSError as e:
+        raise HipLibraryMissing(f"cannot load {path}: {e}") from e
+    lib.mts_last_error.restype = C.c_char_p
+    lib.mts_default_params.argtypes = [C.POINTER(Params)]
+    lib.mts_default_params.restype = None
+    lib.mts_globals.argtypes = [C.POINTER(Params), _D]
+    lib.mts_load_existing_model.argtypes = [C.c_char_p, C.c_int, _I32, _D, _D]
+    lib.mts_create.argtypes = [C.c_int, C.c_int, _I32, _D, _D, C.POINTER(Params), C.POINTER(C.c_void_p)]
+    lib.mts_destroy.argtypes = [C.c_void_p]
+    lib.mts_cell_count.argtypes = [C.c_void_p, _I64]
+    lib.mts_grid_info.argtypes = [C.c_void_p, _I64, _I32, _I32, _I64, _I32]
+    lib.mts_links.argtypes = [C.c_void_p, C.c_int, _I64, _I64, _I32, _I64]
+    lib.mts_set_operators.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.mts_set_body_force.argtypes = [C.c_void_p, C.c_double]
+    lib.mts_initialize.argtypes = [C.c_void_p]
+    lib.mts_step_transport.argtypes = [C.c_void_p, C.c_int]
+    lib.mts_step_adsorption.argtypes = [C.c_void_p, C.c_int]
+    lib.mts_step_coupled.argtypes = [C.c_void_p, C.c_int]
+    lib.mts_residual.argtypes = [C.c_void_p, _D]
+    lib.mts_method.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.mts_field_size.argtypes = [C.c_void_p, C.c_int, _I64]
+    lib.mts_get_field.argtypes = [C.c_void_p, C.c_int, _D]
+    lib.mts_set_field.argtypes = [C.c_void_p, C.c_int, _D]
+    lib.mts_complete_domain.argtypes = [C.c_void_p, _D, _D, _D]
+    lib.mts_diagnostics.argtypes = [C.c_void_p, _D]
+    lib.mts_get_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    lib.mts_synchronize.argtypes = [C.c_void_p]
+    _LIB = lib
+    return lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _check(lib, rc):
+    if rc != 0:
+        raise MtsError(rc, lib.mts_last_error().decode())
+
+
+def default_params(**kw):
+    lib = load_library()
+    p = Params()
+    lib.mts_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "collision" and isinstance(v, str):
+            v = COLL[v]
+        if k == "wall" and isinstance(v, str):
+            v = WALL[v]
+        if k == "precision" and isinstance(v, str):
+            v = PREC[v]
+        setattr(p, k, v)
+    return p
+
+
+def globals_for(**kw):
+    lib = load_library()
+    p = default_params(**kw)
+    out = np.zeros(10)
+    rc = lib.mts_globals(C.byref(p), _p(out, _D))
+    keys = ["rhoin", "mfp", "Cf", "T_lu", "a_alpha_lu", "ZG", "Crho", "Cu", "Cl", "cubic_ok"]
+    return rc, dict(zip(keys, out.tolist()))
+
+
+def load_existing_model(folder, n):
+    lib = load_library()
+    pore = np.zeros((n, n), np.int32)
+    kn = np.zeros((n, n))
+    lp = np.zeros((n, n))
+    folder = folder if folder.endswith("/") else folder + "/"
+    _check(lib, lib.mts_load_existing_model(folder.encode(), n, _p(pore, _I32), _p(kn, _D), _p(lp, _D)))
+    return pore, kn, lp
+
+
+class Lattice:
+    """One LB2D lattice on the GPU (handle of the C ABI)."""
+
+    def __init__(self, pore, kn, locpore, **params):
+        self.lib = load_library()
+        pore = np.ascontiguousarray(pore, dtype=np.int32)
+        kn = np.ascontiguousarray(kn, dtype=np.float64)
+        locpore = np.ascontiguousarray(locpore, dtype=np.float64)
+        self.ny, self.nx = pore.shape
+        self.params = default_params(**params)
+        h = C.c_void_p()
+        _check(self.lib, self.lib.mts_create(self.nx, self.ny, _p(pore, _I32), _p(kn, _D), _p(locpore, _D),
+                                             C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.mts_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ncell(self):
+        n = C.c_int64()
+        _check(self.lib, self.lib.mts_cell_count(self.h, C.byref(n)))
+        return n.value
+
+    def set_operators(self, collision, wall):
+        c = COLL[collision] if isinstance(collision, str) else collision
+        w = WALL[wall] if isinstance(wall, str) else wall
+        _check(self.lib, self.lib.mts_set_operators(self.h, c, w))
+
+    def set_body_force(self, F):
+        _check(self.lib, self.lib.mts_set_body_force(self.h, F))
+
+    def initialize(self):
+        _check(self.lib, self.lib.mts_initialize(self.h))
+
+    def step(self, kind, nsteps):
+        fn = {"transport": self.lib.mts_step_transport, "adsorption": self.lib.mts_step_adsorption,
+              "coupled": self.lib.mts_step_coupled}[kind]
+        _check(self.lib, fn(self.h, int(nsteps)))
+
+    def residual(self):
+        d = C.c_double()
+        _check(self.lib, self.lib.mts_residual(self.h, C.byref(d)))
+        return d.value
+
+    def method(self, name, lvl=1):
+        _check(self.lib, self.lib.mts_method(self.h, METHODS[name], lvl))
+
+    def get(self, name):
+        n = C.c_int64()
+        _check(self.lib, self.lib.mts_field_size(self.h, FIELDS[name], C.byref(n)))
+        out = np.zeros(n.value)
+        _check(self.lib, self.lib.mts_get_field(self.h, FIELDS[name], _p(out, _D)))
+        return out
+
+    def set(self, name, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        _check(self.lib, self.lib.mts_set_field(self.h, FIELDS[name], _p(arr, _D)))
+
+    def complete_domain(self):
+        n = self.nx * self.ny
+        ux, uy, rho = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(self.lib, self.lib.mts_complete_domain(self.h, _p(ux, _D), _p(uy, _D), _p(rho, _D)))
+        return ux, uy, rho
+
+    def diagnostics(self):
+        out = np.zeros(4)
+        _check(self.lib, self.lib.mts_diagnostics(self.h, _p(out, _D)))
+        return out
+
+    def grid(self):
+        n = self.ncell
+        gid = np.zeros(n, np.int64)
+        ij = np.zeros((n, 2), np.int32)
+        lvl = np.zeros(n, np.int32)
+        con = np.zeros((n, 9), np.int64)
+        bnd = np.zeros(n, np.int32)
+        _check(self.lib, self.lib.mts_grid_info(self.h, _p(gid, _I64), _p(ij, _I32), _p(lvl, _I32), _p(con, _I64),
+                                                _p(bnd, _I32)))
+        return dict(gridID=gid, gridIJ=ij, level=lvl, connect=con, bnd=bnd)
+
+    def links(self, which):
+        w = {"stream": 0, "boundary": 1, "periodic": 2}[which]
+        cnt = C.c_int64()
+        _check(self.lib, self.lib.mts_links(self.h, w, None, None, None, C.byref(cnt)))
+        a = np.zeros(cnt.value, np.int64)
+        b = np.zeros(cnt.value, np.int64)
+        c = np.zeros(cnt.value, np.int32)
+        _check(self.lib, self.lib.mts_links(self.h, w, _p(a, _I64), _p(b, _I64), _p(c, _I32), C.byref(cnt)))
+        return a, b, c
+
+    def stream_handle(self):
+        s = C.c_void_p()
+        _check(self.lib, self.lib.mts_get_stream(self.h, C.byref(s)))
+        return s.value
+
+    def synchronize(self):
+        _check(self.lib, self.lib.mts_synchronize(self.h))

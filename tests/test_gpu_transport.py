@@ -1,0 +1,117 @@
+"""Transport step (RMRT + SRBB + periodic) of the HIP library against the CPU
+oracle and the reference's gold (64x64, 100 steps, test/run_tests.sh:15-23)."""
+import numpy as np
+import pytest
+
+import _oracle as O
+from _gpu_util import TRANSPORT, banner, pair, rel_l2
+from mantiis_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+
+def test_grid_and_links_bit_exact(fixture_geometry):
+    banner("grid/links fixture 64x64")
+    pore, kn, lp = fixture_geometry
+    o, g = pair(pore, kn, lp, **TRANSPORT)
+    go, gg = o.grid(), g.grid()
+    for k in ("gridID", "gridIJ", "level", "connect", "bnd"):
+        assert np.array_equal(go[k], gg[k]), k
+    for which in ("stream", "boundary", "periodic"):
+        a, b = o.links(which), g.links(which)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2]), which
+        if which != "boundary":
+            assert np.array_equal(a[1], b[1]), which
+    assert np.array_equal(o.get("taus"), g.get("taus"))
+
+
+def test_links_bit_exact_on_seamed_packing():
+    """Fluid across both seams and at the corners: y-seam drop, x-seam periodic."""
+    banner("links periodic disk packing 96x96")
+    pore = G.disk_packing(96, 0.6, 3, 8, seed=3)
+    kn, lp = G.uniform_kn(pore, 0.1, 0.773085)
+    o, g = pair(pore, kn, lp, **TRANSPORT)
+    for which in ("stream", "boundary", "periodic"):
+        a, b = o.links(which), g.links(which)
+        assert len(a[0]) == len(b[0]), which
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2]), which
+    assert len(o.links("periodic")[0]) > 0
+
+
+def test_transport_gold_fp64(fixture_geometry, gold):
+    banner("transport fixture FP64 100 x 1 step")
+    pore, kn, lp = fixture_geometry
+    o, g = pair(pore, kn, lp, **TRANSPORT)
+    o.run("transport", 100, 1e-10)
+    hist = []
+    for _ in range(100):
+        g.step("transport", 1)
+        hist.append(g.residual())
+    ux, uy, rho = g.complete_domain()
+    oux, ouy, orho = o.complete_domain()
+    for name, a, b in (("ux", ux, oux), ("uy", uy, ouy), ("rho", rho, orho)):
+        assert np.abs(a - gold[("transport", name)]).max() <= 1e-10, name
+        d = np.abs(a - b).max()
+        print(name, "max|gpu-oracle| =", d, "rel-l2 =", rel_l2(a, b))
+        assert rel_l2(a, b) <= 5e-13, name
+    assert np.allclose(hist, o.diff_history(), rtol=1e-11, atol=0)
+    f_o, f_g = o.get("f"), g.get("f")
+    print("f max abs diff", np.abs(f_o - f_g).max())
+    assert np.abs(f_o - f_g).max() <= 1e-16
+
+
+def test_multistep_call_equals_single_steps(fixture_geometry):
+    banner("transport fixture FP64 one call of 100 steps")
+    pore, kn, lp = fixture_geometry
+    o, g1 = pair(pore, kn, lp, **TRANSPORT)
+    from mantiis_amd.hip import Lattice
+    g2 = Lattice(pore, kn, lp, precision="fp64", **TRANSPORT)
+    for _ in range(37):
+        g1.step("transport", 1)
+    g2.step("transport", 37)
+    assert np.array_equal(g1.get("f"), g2.get("f"))
+    assert g1.residual() == g2.residual()
+    o.run("transport", 37, 0.0)
+    assert abs(g2.residual() - o.diff_history()[-1]) <= 1e-12 * o.diff_history()[-1]
+
+
+def test_transport_fp32_deviation(fixture_geometry):
+    banner("transport fixture FP32 deviation storage 100 steps")
+    pore, kn, lp = fixture_geometry
+    o, g = pair(pore, kn, lp, precision="fp32dev", **TRANSPORT)
+    o.run("transport", 100, 0.0)
+    g.step("transport", 100)
+    ux, uy, rho = g.complete_domain()
+    oux, ouy, orho = o.complete_domain()
+    e = dict(ux=rel_l2(ux, oux), uy=rel_l2(uy, ouy), rho=rel_l2(rho, orho))
+    print("FP32-dev rel-l2 vs FP64 oracle", e)
+    assert e["rho"] < 1e-8 and e["ux"] < 1e-3 and e["uy"] < 5e-3
+
+
+@pytest.mark.parametrize("wall", [0, 1])
+@pytest.mark.parametrize("coll", [0, 1])
+def test_transport_operators(fixture_geometry, coll, wall):
+    banner(f"transport fixture FP64 collision={coll} wall={wall} 20 steps")
+    pore, kn, lp = fixture_geometry
+    o, g = pair(pore, kn, lp, collision=coll, wall=wall, **TRANSPORT)
+    o.run("transport", 20, 0.0)
+    g.step("transport", 20)
+    d = np.abs(o.get("f") - g.get("f")).max()
+    print("max |f| diff", d)
+    assert d <= 1e-15
+
+
+def test_transport_disk_packing_fp64():
+    """A seeded 256^2 disk packing (r ~ U[4,12], porosity 0.6), solid rows at
+    y = 0 and N-1 so that no link is lost at the y-seam; Kn = 0.1."""
+    banner("transport disk packing 256x256 FP64 200 steps")
+    pore = G.disk_packing(256, 0.6, 4, 12, seed=1, solid_rows=True)
+    rc = O.load()
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    o, g = pair(pore, kn, lp, dx=1e-9, Fbody=1e-6, T=300.0, P=2e6)
+    o.run("transport", 200, 0.0)
+    g.step("transport", 200)
+    ux, uy, rho = g.complete_domain()
+    oux, ouy, orho = o.complete_domain()
+    print("rel-l2 ux", rel_l2(ux, oux), "rho", rel_l2(rho, orho))
+    assert rel_l2(ux, oux) < 5e-10 and rel_l2(rho, orho) < 1e-14

@@ -1,0 +1,142 @@
+"""Closed-form checks that pin the oracle's constants and collision operators
+independently of the gold files."""
+import numpy as np
+import pytest
+
+import _oracle as O
+
+EX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+EY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def test_constants_match_reference_printout():
+    """Values the reference prints at start-up (mantiisApp.hpp:146-155) for the
+    regression settings: rho_in, mfp and Cf."""
+    lib = O.load()
+    out = np.zeros(10)
+    lib.orc_globals_for(1e-9, 300.0, 2e6, out.ctypes.data_as(O._D))
+    assert abs(out[0] - 0.235554544075989) < 1e-15
+    assert abs(out[1] - 7.73085e-10) < 1e-15
+    assert abs(out[2] - 7.72066669163199e16) / 7.72066669163199e16 < 1e-14
+    lib.orc_globals_for(12e-9, 300.0, 2e6, out.ctypes.data_as(O._D))
+    assert abs(out[2] - 6.43388890969332e15) / 6.43388890969332e15 < 1e-14
+    assert abs(out[0] - 0.235554544075989) < 1e-15  # rho_in does not depend on dx
+
+
+@pytest.mark.parametrize("T,P", [(300.0, 2e6), (350.0, 5e6), (250.0, 1e6)])
+def test_gas_root_solves_pr_eos(T, P):
+    """Z is a root of Z^3 + (B-1)Z^2 + (A-3B^2-2B)Z + (B^3+B^2-AB) = 0 (PR-EOS)."""
+    lib = O.load()
+    out = np.zeros(10)
+    lib.orc_globals_for(1e-9, T, P, out.ctypes.data_as(O._D))
+    Z = out[5]
+    R, Tc, Pc, om = 8.31446, 190.564, 4.5992e6, 0.011
+    k = 0.37464 + 1.54226 * om - 0.26992 * om ** 2
+    alpha = (1 + k * (1 - np.sqrt(T / Tc))) ** 2
+    A = 0.45724 * (R * Tc) ** 2 / Pc * alpha * P / (R * R * T * T)
+    B = 0.07780 * R * Tc / Pc * P / (R * T)
+    res = Z ** 3 + (B - 1) * Z ** 2 + (A - 3 * B * B - 2 * B) * Z + (B ** 3 + B ** 2 - A * B)
+    assert abs(res) < 1e-13
+    assert 0.5 < Z < 1.0  # gas-like compressibility
+
+
+def _state(seed=0):
+    pore = np.ones((8, 8), np.int32)
+    pore[0, :] = 0
+    pore[-1, :] = 0
+    kn = np.where(pore == 1, 0.2, 0.0)
+    lp = np.where(pore == 1, 4.0, 0.0)
+    o = O.Oracle(pore, kn, lp)
+    rng = np.random.default_rng(seed)
+    n = o.ncell
+    rho = 0.2 + 0.05 * rng.random(n)
+    ux = 0.05 * (rng.random(n) - 0.5)
+    uy = 0.05 * (rng.random(n) - 0.5)
+    return o, n, rho, ux, uy
+
+
+def test_equilibrium_moments():
+    """sum feq = rho, sum feq c = rho u, sum feq cc = rho (cs^2 I + uu)."""
+    o, n, rho, ux, uy = _state()
+    o.set("rho", rho); o.set("ux", ux); o.set("uy", uy); o.set("usq", ux * ux + uy * uy)
+    o.method("equilibrium")
+    fe = o.get("feq").reshape(n, 9)
+    assert np.allclose(fe.sum(1), rho, rtol=1e-14)
+    assert np.allclose(fe @ EX, rho * ux, atol=1e-15)
+    assert np.allclose(fe @ EY, rho * uy, atol=1e-15)
+    assert np.allclose(fe @ (EX * EX), rho * (1 / 3 + ux * ux), rtol=1e-13)
+    assert np.allclose(fe @ (EX * EY), rho * ux * uy, atol=1e-15)
+
+
+def test_regularized_nonequilibrium_has_no_mass_or_momentum():
+    o, n, rho, ux, uy = _state(1)
+    o.set("rho", rho); o.set("ux", ux); o.set("uy", uy); o.set("usq", ux * ux + uy * uy)
+    o.method("equilibrium")
+    rng = np.random.default_rng(5)
+    f = o.get("feq") * (1 + 0.01 * rng.standard_normal(9 * n))
+    o.set("f", f)
+    o.method("regularize")
+    fn = o.get("fneq").reshape(n, 9)
+    d = (f - o.get("feq")).reshape(n, 9)
+    assert np.abs(fn.sum(1)).max() < 1e-16
+    assert np.abs(fn @ EX).max() < 1e-16 and np.abs(fn @ EY).max() < 1e-16
+    # second moments of fneq equal those of f - feq (the Hermite projection keeps Pi)
+    for a, b in ((EX, EX), (EX, EY), (EY, EY)):
+        assert np.allclose(fn @ (a * b), d @ (a * b), atol=1e-16)
+
+
+def test_mrt_inverse():
+    M = np.array([[1] * 9, [-4, -1, -1, -1, -1, 2, 2, 2, 2], [4, -2, -2, -2, -2, 1, 1, 1, 1],
+                  [0, 1, 0, -1, 0, 1, -1, -1, 1], [0, -2, 0, 2, 0, 1, -1, -1, 1], [0, 0, 1, 0, -1, 1, 1, -1, -1],
+                  [0, 0, -2, 0, 2, 1, 1, -1, -1], [0, 1, -1, 1, -1, 0, 0, 0, 0], [0, 0, 0, 0, 0, 1, -1, 1, -1]], float)
+    # the 15-decimal truncated inverse used by the reference is an inverse to 1e-14
+    Minv = np.round(np.linalg.inv(M), 15)
+    assert np.abs(Minv @ M - np.eye(9)).max() < 1e-14
+
+
+def test_collision_operators_moments():
+    """With every MRT rate 1 the collision maps f to feq, as BGK with tau 1 does
+    (SRTCollision, coreLB.hpp:678-691).  A cell with tau_s = tau_q = tau_d = 1
+    is not reachable by the rates' formulas, so compare mass conservation and the
+    relaxation of the non-equilibrium stress instead, plus SRT == feq."""
+    o, n, rho, ux, uy = _state(2)
+    o.set("rho", rho); o.set("ux", ux); o.set("uy", uy); o.set("usq", ux * ux + uy * uy)
+    o.method("equilibrium")
+    rng = np.random.default_rng(9)
+    f0 = o.get("feq") * (1 + 0.01 * rng.standard_normal(9 * n))
+    o.set("f", f0)
+    o.method("SRTCollision")
+    assert np.allclose(o.get("f"), o.get("feq"), atol=1e-17)
+    o.set("f", f0)
+    o.method("MRTCollisionNonRegularized")
+    f1 = o.get("f").reshape(n, 9)
+    # the density moment relaxes with rate 1: mass after collision is sum feq = rho
+    assert np.allclose(f1.sum(1), rho, rtol=1e-14)
+    o.set("f", f0)
+    o.method("MRTCollisionRegularized")
+    f2 = o.get("f").reshape(n, 9)
+    fe = o.get("feq").reshape(n, 9)
+    assert np.allclose(f2.sum(1), fe.sum(1), rtol=1e-14)
+    assert np.allclose(f2 @ EX, fe @ EX, atol=1e-16)
+    # the shear stress relaxes by (1 - 1/tau_s)
+    taus = o.get("taus")
+    pxy_in = o.get("fneq").reshape(n, 9) @ (EX * EY)
+    pxy_out = (f2 - fe) @ (EX * EY)
+    assert np.allclose(pxy_out, (1 - 1 / taus) * pxy_in, atol=1e-16)
+
+
+def test_mass_conserved_with_bounce_back_and_periodic():
+    """Closed channel (solid rows), BB walls: total mass is invariant."""
+    pore = np.ones((16, 16), np.int32)
+    pore[0, :] = 0
+    pore[-1, :] = 0
+    pore[6:9, 5:8] = 0
+    kn = np.where(pore == 1, 0.1, 0.0)
+    lp = np.where(pore == 1, 7.73085, 0.0)
+    for wall in (0, 2):
+        o = O.Oracle(pore, kn, lp, Fbody=1e-5)
+        o.set_operators(2, wall)
+        m0 = o.get("f").sum()
+        o.run("transport", 50, 0.0)
+        assert abs(o.get("f").sum() - m0) / m0 < 1e-13
