@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X D2Q9 step (one JSON line on stdout).
+
+Default workload: the Shan-Chen adsorption configuration -- a 2048x2048
+seeded disk packing at porosity 0.5, fluid-solid coupling G_ads drawn from
+U[-1.5, -0.5], initial density uniform in [0.1, 0.5] with 1% seeded noise,
+FP32 deviation storage with an FP64 pseudopotential.  A "step" is one
+adsorption time step of the whole lattice (Shan-Chen forces, MRT + Guo
+source, bounce-back, periodic and generalized-periodic boundaries).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|transport|c1] [--no-cpu]
+
+value = fluid-cell updates per second (MLUPS) over exactly K steps timed with
+HIP events on the library's stream, inputs resident in HBM.  The roofline
+entry covers the fused Shan-Chen kernel: algorithmic bytes per fluid-cell
+update (88 B: 9 FP32 populations read and written, 8 B psi written and read)
+times the fluid cells of one launch, over that kernel's average launch time
+measured in the same run.  The CPU baseline is the oracle (the C++
+restatement of the reference at the reference's -Ofast -fopenmp flags) timed
+on a bounded sample of the same lattice.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def c3_inputs(n=2048, seed=2027):
+    from mantiis_amd import geometry as G
+    rng = np.random.default_rng(seed)
+    pore = G.disk_packing(n, 0.5, 4, 12, seed=seed)
+    G_ads = float(rng.uniform(-1.5, -0.5))
+    rho0 = float(rng.uniform(0.1, 0.5))
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    rho = rho0 * (1.0 + 0.01 * rng.standard_normal(pore.shape))
+    return pore, kn, lp, G_ads, rho0, rho
+
+
+def transport_inputs(n=1024, seed=2026):
+    """C2-distribution geometry (porosity U[0.4, 0.8], Kn = 0.5) run single-block."""
+    from mantiis_amd import geometry as G
+    rng = np.random.default_rng(seed)
+    phi = float(rng.uniform(0.4, 0.8))
+    pore = G.disk_packing(n, phi, 4, 12, seed=seed)
+    kn, lp = G.uniform_kn(pore, 0.5, 0.7730846760658538)
+    return pore, kn, lp
+
+
+def w9():
+    return np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def make_c3(precision="fp32dev"):
+    from mantiis_amd.hip import Lattice
+    pore, kn, lp, G_ads, rho0, rho = c3_inputs()
+    lat = Lattice(pore, kn, lp, precision=precision, collision=1, wall=0, dx=1e-9, Fbody=0.0, Gfs=G_ads)
+    fl = pore.reshape(-1) == 1
+    r = rho.reshape(-1)[fl]
+    lat.set("f", (r[:, None] * w9()[None, :]).reshape(-1))
+    lat.set("rho", r)
+    return lat, dict(pore=pore, kn=kn, lp=lp, G_ads=G_ads, rho0=rho0, rho=rho)
+
+
+def time_steps(lat, kind, steps, warmup):
+    from mantiis_amd.hiptime import StreamTimer
+    t = StreamTimer(lat.stream_handle())
+    if warmup > 0:
+        lat.step(kind, warmup)
+    lat.synchronize()
+    t.start()
+    lat.step(kind, steps)
+    ms = t.stop()
+    # one-step call: collide-only + stream-only passes (+ residual) without fused passes
+    t.start()
+    lat.step(kind, 1)
+    ms1 = t.stop()
+    t.close()
+    return ms, ms1
+
+
+def cpu_baseline(inputs, sample_steps):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    o = O.Oracle(inputs["pore"], inputs["kn"], inputs["lp"], dx=1e-9, Fbody=0.0, fast=True)
+    o.set_interaction(-1.0, inputs["G_ads"])
+    fl = inputs["pore"].reshape(-1) == 1
+    r = inputs["rho"].reshape(-1)[fl]
+    o.set("f", (r[:, None] * w9()[None, :]).reshape(-1))
+    o.set("rho", r)
+    o.run("adsorption", 1, 0.0)  # warm-up
+    t0 = time.perf_counter()
+    o.run("adsorption", sample_steps, 0.0)
+    dt = time.perf_counter() - t0
+    n = o.ncell
+    threads = int(O.load(fast=True).orc_threads())
+    o.close()
+    return dict(value=round(n * sample_steps / dt / 1e6, 4), unit="MLUPS", cores=threads, kind="port",
+                sample=f"{sample_steps} adsorption steps of the same 2048^2 lattice ({n} fluid cells) after 1 warm-up "
+                       f"step; oracle built -Ofast -fopenmp, OMP threads {threads} (forces and Guo source loops are "
+                       f"serial as in the reference)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="c3", choices=["c3", "transport", "c1"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        return 2
+    if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
+        print("bench.py is a single-process, single-GPU benchmark.", file=sys.stderr)
+        return 2
+    if args.steps < 2:
+        print("--steps must be at least 2", file=sys.stderr)
+        return 2
+
+    if args.workload == "c3":
+        lat, inputs = make_c3()
+        kind = "adsorption"
+        bytes_per_cell = 88
+        workload = ("Shan-Chen adsorption, 2048x2048 seeded disk packing porosity 0.5, "
+                    f"G_ads={inputs['G_ads']:.4f} (U[-1.5,-0.5]), rho0={inputs['rho0']:.4f} (U[0.1,0.5]) +1% noise, "
+                    "MRT + Guo source + bounce-back, FP32 deviation storage, FP64 psi")
+        kernel = "k_shanchen<float,1,MRT,BB,genperiodic>"
+        dtype = "fp32 (populations) / fp64 (psi, moments)"
+    elif args.workload == "transport":
+        from mantiis_amd.hip import Lattice
+        pore, kn, lp = transport_inputs()
+        lat = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6)
+        kind, bytes_per_cell, inputs = "transport", 72, None
+        workload = "transport RMRT+SRBB, 1024x1024 disk packing (porosity U[0.4,0.8]), Kn=0.5, single-block, FP32 deviation"
+        kernel = "k_transport<float,1,RMRT,SRBB>"
+        dtype = "fp32 (populations)"
+    else:
+        from mantiis_amd import geometry as G
+        from mantiis_amd.hip import Lattice
+        pore = G.disk_packing(256, 0.6, 4, 12, seed=1)
+        kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+        lat = Lattice(pore, kn, lp, precision="fp64", collision=2, wall=2, dx=1e-9, Fbody=1e-6)
+        kind, bytes_per_cell, inputs = "transport", 144, None
+        workload = "transport RMRT+SRBB, 256x256 disk packing r~U[4,12] porosity 0.6, Kn=0.1, FP64"
+        kernel = "k_transport<double,1,RMRT,SRBB>"
+        dtype = "fp64"
+
+    n = lat.ncell
+    ms, ms1 = time_steps(lat, kind, args.steps, args.warmup)
+    mlups = n * args.steps / (ms / 1e3) / 1e6
+    t_kernel_ms = (ms - ms1) / (args.steps - 1)
+    achieved = bytes_per_cell * n / (t_kernel_ms / 1e3) / 1e9
+    diag = lat.diagnostics()
+
+    cpu = None
+    if not args.no_cpu and inputs is not None:
+        cpu = cpu_baseline(inputs, args.cpu_steps)
+
+    out = {
+        "metric": "MLUPS (fluid-cell updates per second) & %HBM roofline",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded disk packing)",
+        "config": {"workload": workload, "lattice": [lat.nx, lat.ny], "fluid_cells": n, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_cell": bytes_per_cell, "kernel_ms": round(t_kernel_ms, 6)},
+        "cpu_baseline": cpu,
+        "mass": float(diag[0]),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -69,7 +69,9 @@ typedef struct mts_params {
     int precision;   /* MTS_FP64 / MTS_FP32DEV                 default FP64  */
     int multiblock;  /* 0 single-block grid (the only mode in this build)   */
     int device;      /* HIP device ordinal                     default 0     */
-    int reserved;
+    int periodic_y;  /* 0: links crossing only the y-seam are dropped, as at
+                      * coreLB.hpp:301-331 (default); 1: they are periodic
+                      * links like the x-seam ones (true y-periodicity)      */
 } mts_params;
 
 /* Fills the defaults above. */

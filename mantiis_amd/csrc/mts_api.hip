@@ -85,6 +85,7 @@ struct mts_sim_s {
         c.a_alpha_lu = ph.a_alpha_lu;
         c.gpIn = geo.gpIn;
         c.gpOut = geo.gpOut;
+        c.one_param = geo.params.size() == 1;
         return c;
     }
     template <class T>
@@ -149,6 +150,7 @@ void mts_default_params(mts_params* p) {
     p->precision = MTS_FP64;
     p->multiblock = 0;
     p->device = 0;
+    p->periodic_y = 0;
 }
 
 const char* mts_last_error(void) { return g_err.c_str(); }
@@ -215,7 +217,7 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
         return fail(MTS_ERR_PHYSICS, "PR-EOS: no single positive real root at this (T, P)");
     }
     std::string err;
-    int rc = build_single_grid(nx, ny, pore, kn, locpore, s->ph, s->geo, err);
+    int rc = build_single_grid(nx, ny, pore, kn, locpore, s->ph, prm.periodic_y != 0, s->geo, err);
     if (rc) {
         delete s;
         return fail(rc, err);

@@ -20,6 +20,7 @@ struct LaunchCtx {
     int coll, wall;
     double rho0, rhoin, Fbody, Gff, Gfs, T_lu, a_alpha_lu;
     int64_t gpIn, gpOut;
+    bool one_param;  // a single (tau_s, Kn) pair: the per-site index is not read
 };
 
 struct TransportSpec {
@@ -40,6 +41,9 @@ struct ScSpec {
 };
 
 void upload_icsr(hipStream_t st);
+// FP32 production kernels (mts_fast.inc)
+int launch_transport32(const LaunchCtx& c, int mode, const TransportSpec& s);
+int launch_shanchen32(const LaunchCtx& c, int kind, int mode, const ScSpec& s);
 // mode 0 collide-only (fcan -> Aout), 1 fused (Ain -> Aout), 2 stream-only (Ain -> fcan)
 int launch_transport(const LaunchCtx& c, int mode, const TransportSpec& s);
 // kind 0 adsorption, 1 coupled; mode as above

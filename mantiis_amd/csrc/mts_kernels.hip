@@ -84,9 +84,10 @@ struct Lat {
         ys = ys < 0 ? ys + ny : (ys >= ny ? ys - ny : ys);
         return (int64_t)ys * nx + xs;
     }
-    // slot j of site x is fed across the x-seam (a periodic link)
-    __device__ __forceinline__ bool xseam_in(int x, int j) const {
-        return (x == 0 && ex(j) == 1) || (x == nx - 1 && ex(j) == -1);
+    // slot j of site (x, y) is fed across a seam (a periodic link when its IN bit is set)
+    __device__ __forceinline__ bool seam_in(int x, int y, int j) const {
+        return (x == 0 && ex(j) == 1) || (x == nx - 1 && ex(j) == -1) || (y == 0 && ey(j) == 1) ||
+               (y == ny - 1 && ey(j) == -1);
     }
 };
 
@@ -284,10 +285,10 @@ __device__ __forceinline__ void gather(const S* __restrict__ A, const Lat& L, in
             }
         }
         // periodic links overwrite after the wall rule (coreLB.hpp:969-977)
-        if (w & XSEAM_BIT) {
+        if (w & SEAM_BIT) {
 #pragma unroll
             for (int j = 1; j < 9; j++)
-                if (L.xseam_in(x, j) && ((w >> (IN_SHIFT + j - 1)) & 1u)) {
+                if (L.seam_in(x, y, j) && ((w >> (IN_SHIFT + j - 1)) & 1u)) {
                     f[j] = keep[j];
                     base[j] = 0.0;
                 }
@@ -704,11 +705,7 @@ int launch_transport(const LaunchCtx& c, int mode, const TransportSpec& s) {
     a.uy = s.uy;
     a.usq = s.usq;
     a.usq_old = s.usq_old;
-    if (c.fp32) {
-        if (mode == 0) return dispatch_t<float, 0>(c, a, c.coll, c.wall);
-        if (mode == 1) return dispatch_t<float, 1>(c, a, c.coll, c.wall);
-        return dispatch_t<float, 2>(c, a, c.coll, c.wall);
-    }
+    if (c.fp32) return launch_transport32(c, mode, s);
     if (mode == 0) return dispatch_t<double, 0>(c, a, c.coll, c.wall);
     if (mode == 1) return dispatch_t<double, 1>(c, a, c.coll, c.wall);
     return dispatch_t<double, 2>(c, a, c.coll, c.wall);
@@ -763,8 +760,8 @@ int launch_shanchen(const LaunchCtx& c, int kind, int mode, const ScSpec& s) {
     a.Fx = s.Fx;
     a.Fy = s.Fy;
     a.psi_negative = s.psi_negative;
-    if (c.fp32) sc_dispatch<float>(c, kind, mode, a);
-    else sc_dispatch<double>(c, kind, mode, a);
+    if (c.fp32) return launch_shanchen32(c, kind, mode, s);
+    sc_dispatch<double>(c, kind, mode, a);
     return 0;
 }
 
@@ -795,3 +792,4 @@ void launch_macro(const LaunchCtx& c, const double* fcan, double* rho, double* u
 
 }  // namespace mts
 #include "mts_phase.inc"
+#include "mts_fast.inc"

@@ -46,7 +46,7 @@ class MtsError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("dx", C.c_double), ("Fbody", C.c_double), ("T", C.c_double), ("P", C.c_double),
                 ("Gff", C.c_double), ("Gfs", C.c_double), ("collision", C.c_int), ("wall", C.c_int),
-                ("precision", C.c_int), ("multiblock", C.c_int), ("device", C.c_int), ("reserved", C.c_int)]
+                ("precision", C.c_int), ("multiblock", C.c_int), ("device", C.c_int), ("periodic_y", C.c_int)]
 
 
 _LIB = None

@@ -357,6 +357,9 @@ struct Sim {
     double diff = 1.0;
     std::vector<double> diff_over_time;
     int collision = 2, wall = 2;
+    // false: the reference's seam rule (y-seam-only links dropped,
+    // coreLB.hpp:301-331); true: y-seam links periodic like x-seam links.
+    bool periodicY = false;
 
     std::vector<double> rho, rho_old, ux, uy, Fx, Fy, uxsq, uysq, usq, usq_old, usqMinusOld;
     std::vector<double> f, fb, ftemp, feq, fneq, psi, taus;
@@ -404,7 +407,7 @@ struct Sim {
                             boundaryID.push_back(i); boundaryIC.push_back(ic);
                             bfIC[i].push_back(ic);
                         }
-                    } else if (xp) {
+                    } else if (xp || (periodicY && yp)) {
                         periodicID.push_back(i); periodicI2.push_back(c); periodicIC.push_back(ic);
                         sfIC[i].push_back(ic); sfI2[i].push_back(c);
                     }
@@ -848,9 +851,10 @@ const char* orc_last_error() { return g_err.c_str(); }
 // Inputs are row-major [y][x] (the in-memory Shape::domain layout after the
 // transposed load of shape.hpp:253-285).  multiblock=1 runs the quadtree path.
 void* orc_create(int N, const int* pore, const double* kn, const double* locpore, double dx,
-                 double Fbody, double T, double P, const char* tables_dir, int multiblock) {
+                 double Fbody, double T, double P, const char* tables_dir, int multiblock, int periodic_y) {
     try {
         Sim* s = new Sim();
+        s->periodicY = periodic_y != 0;
         s->g.Cl = dx; s->g.Fbody = Fbody; s->g.T_phy = T; s->g.P_phy = P;
         s->g.recompute();
         orc::loadTables(tables_dir, s->T);

@@ -29,7 +29,7 @@ def load(fast=False):
     lib = C.CDLL(path)
     lib.orc_create.restype = _P
     lib.orc_create.argtypes = [C.c_int, _I, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double,
-                               C.c_char_p, C.c_int]
+                               C.c_char_p, C.c_int, C.c_int]
     lib.orc_destroy.argtypes = [_P]
     lib.orc_last_error.restype = C.c_char_p
     lib.orc_globals.argtypes = [_P, _D]
@@ -83,14 +83,14 @@ class Oracle:
     RUN_KIND = {"transport": 0, "adsorption": 1, "coupled": 2}
 
     def __init__(self, pore, kn, locpore, dx=1e-9, Fbody=1e-8, T=300.0, P=2e6, multiblock=False,
-                 fast=False):
+                 fast=False, periodic_y=False):
         self.lib = load(fast)
         self.n = int(pore.shape[0])
         self._pore = np.ascontiguousarray(pore, dtype=np.int32)
         self._kn = np.ascontiguousarray(kn, dtype=np.float64)
         self._lp = np.ascontiguousarray(locpore, dtype=np.float64)
         h = self.lib.orc_create(self.n, _ptr(self._pore, _I), _ptr(self._kn, _D), _ptr(self._lp, _D),
-                                dx, Fbody, T, P, TABLES.encode(), 1 if multiblock else 0)
+                                dx, Fbody, T, P, TABLES.encode(), 1 if multiblock else 0, 1 if periodic_y else 0)
         if not h:
             raise RuntimeError(self.lib.orc_last_error().decode())
         self.h = h

@@ -65,4 +65,23 @@ def test_adsorption_fp32_deviation(fixture_geometry):
     oux, ouy, orho = o.complete_domain()
     e = dict(ux=rel_l2(ux, oux), uy=rel_l2(uy, ouy), rho=rel_l2(rho, orho))
     print("FP32-dev rel-l2 vs FP64 oracle", e)
-    assert e["rho"] < 1e-5 and e["ux"] < 1e-3 and e["uy"] < 1e-3
+    assert e["rho"] < 1e-5 and e["ux"] < 2e-3 and e["uy"] < 2e-3
+
+
+def test_shanchen_fp32_large_tile_edges():
+    """FP32 Shan-Chen on a lattice whose size is not a multiple of the 64x16
+    tile (edge tiles, halo wrap) against the FP64 path of the library."""
+    banner("adsorption 200x72 fp32dev vs fp64 50 steps")
+    from mantiis_amd import geometry as G
+    from mantiis_amd.hip import Lattice
+    pore = G.disk_packing(200, 0.6, 3, 8, seed=4)[:72, :].copy()
+    pore = G.sanitize(pore)
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    a = Lattice(pore, kn, lp, precision="fp64", collision=1, wall=0, periodic_y=1, **ADSORPTION)
+    b = Lattice(pore, kn, lp, precision="fp32dev", collision=1, wall=0, periodic_y=1, **ADSORPTION)
+    a.step("adsorption", 50)
+    b.step("adsorption", 50)
+    for name in ("rho", "ux", "uy", "psi"):
+        e = rel_l2(b.get(name), a.get(name))
+        print(name, e)
+        assert e < 2e-3, name

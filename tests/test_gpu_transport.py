@@ -85,7 +85,7 @@ def test_transport_fp32_deviation(fixture_geometry):
     oux, ouy, orho = o.complete_domain()
     e = dict(ux=rel_l2(ux, oux), uy=rel_l2(uy, ouy), rho=rel_l2(rho, orho))
     print("FP32-dev rel-l2 vs FP64 oracle", e)
-    assert e["rho"] < 1e-8 and e["ux"] < 1e-3 and e["uy"] < 5e-3
+    assert e["rho"] < 1e-7 and e["ux"] < 1e-3 and e["uy"] < 5e-3
 
 
 @pytest.mark.parametrize("wall", [0, 1])
@@ -115,3 +115,40 @@ def test_transport_disk_packing_fp64():
     oux, ouy, orho = o.complete_domain()
     print("rel-l2 ux", rel_l2(ux, oux), "rho", rel_l2(rho, orho))
     assert rel_l2(ux, oux) < 5e-10 and rel_l2(rho, orho) < 1e-14
+
+
+def test_physics_mode_y_periodic_fp64():
+    """periodic_y=1: links across the y-seam are periodic (not dropped)."""
+    banner("transport physics mode (y-periodic) 128x128 FP64 100 steps")
+    pore = G.disk_packing(128, 0.6, 4, 10, seed=11)
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-6, periodic_y=True)
+    from mantiis_amd.hip import Lattice
+    g = Lattice(pore, kn, lp, precision="fp64", dx=1e-9, Fbody=1e-6, periodic_y=1)
+    for which in ("stream", "boundary", "periodic"):
+        a, b = o.links(which), g.links(which)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2]), which
+    m0 = g.diagnostics()[0]
+    o.run("transport", 100, 0.0)
+    g.step("transport", 100)
+    d = np.abs(o.get("f") - g.get("f")).max()
+    print("max |f| diff", d)
+    assert d <= 1e-15
+    assert abs(g.diagnostics()[0] - m0) / m0 < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32dev"])
+def test_mass_conservation_long_run(prec):
+    """Bounce-back walls + periodic seams conserve mass; 2000 steps, 512^2."""
+    banner(f"transport mass conservation 512x512 {prec} 2000 steps")
+    from mantiis_amd.hip import Lattice
+    pore = G.disk_packing(512, 0.5, 4, 12, seed=5)
+    kn, lp = G.uniform_kn(pore, 0.5, 0.7730846760658538)
+    g = Lattice(pore, kn, lp, precision=prec, dx=1e-9, Fbody=1e-6, periodic_y=1)
+    m0 = g.diagnostics()[0]
+    g.step("transport", 2000)
+    d = g.diagnostics()
+    print("mass drift", (d[0] - m0) / m0, "mean jx", d[1] / d[3])
+    assert np.isfinite(d).all()
+    assert abs(d[0] - m0) / m0 < (1e-12 if prec == "fp64" else 1e-6)
+    assert d[1] > 0  # the body force drives a net +x flux
