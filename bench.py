@@ -61,7 +61,7 @@ def w9():
 def make_c3(precision="fp32dev"):
     from mantiis_amd.hip import Lattice
     pore, kn, lp, G_ads, rho0, rho = c3_inputs()
-    lat = Lattice(pore, kn, lp, precision=precision, collision=1, wall=0, dx=1e-9, Fbody=0.0, Gfs=G_ads)
+    lat = Lattice(pore, kn, lp, precision=precision, collision=1, wall=0, dx=1e-9, Fbody=0.0, Gfs=G_ads, periodic_y=1)
     fl = pore.reshape(-1) == 1
     r = rho.reshape(-1)[fl]
     lat.set("f", (r[:, None] * w9()[None, :]).reshape(-1))
@@ -89,7 +89,7 @@ def time_steps(lat, kind, steps, warmup):
 def cpu_baseline(inputs, sample_steps):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
-    o = O.Oracle(inputs["pore"], inputs["kn"], inputs["lp"], dx=1e-9, Fbody=0.0, fast=True)
+    o = O.Oracle(inputs["pore"], inputs["kn"], inputs["lp"], dx=1e-9, Fbody=0.0, fast=True, periodic_y=True)
     o.set_interaction(-1.0, inputs["G_ads"])
     fl = inputs["pore"].reshape(-1) == 1
     r = inputs["rho"].reshape(-1)[fl]
@@ -116,6 +116,9 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c3", "transport", "c1"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--n", type=int, default=0, help="lattice side of the transport workload (default 1024)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a plain 1 GiB device copy (torch) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -128,6 +131,17 @@ def main():
         print("--steps must be at least 2", file=sys.stderr)
         return 2
 
+    if args.copy_probe:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "launches": 5}))
+        return 0
     if args.workload == "c3":
         lat, inputs = make_c3()
         kind = "adsorption"
@@ -139,10 +153,11 @@ def main():
         dtype = "fp32 (populations) / fp64 (psi, moments)"
     elif args.workload == "transport":
         from mantiis_amd.hip import Lattice
-        pore, kn, lp = transport_inputs()
-        lat = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6)
+        pore, kn, lp = transport_inputs(args.n or 1024)
+        lat = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1)
         kind, bytes_per_cell, inputs = "transport", 72, None
-        workload = "transport RMRT+SRBB, 1024x1024 disk packing (porosity U[0.4,0.8]), Kn=0.5, single-block, FP32 deviation"
+        workload = (f"transport RMRT+SRBB, {lat.nx}x{lat.ny} disk packing (porosity U[0.4,0.8]), Kn=0.5, torus, "
+                    "single-block, FP32 deviation")
         kernel = "k_transport<float,1,RMRT,SRBB>"
         dtype = "fp32 (populations)"
     else:
@@ -150,7 +165,7 @@ def main():
         from mantiis_amd.hip import Lattice
         pore = G.disk_packing(256, 0.6, 4, 12, seed=1)
         kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
-        lat = Lattice(pore, kn, lp, precision="fp64", collision=2, wall=2, dx=1e-9, Fbody=1e-6)
+        lat = Lattice(pore, kn, lp, precision="fp64", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1)
         kind, bytes_per_cell, inputs = "transport", 144, None
         workload = "transport RMRT+SRBB, 256x256 disk packing r~U[4,12] porosity 0.6, Kn=0.1, FP64"
         kernel = "k_transport<double,1,RMRT,SRBB>"

@@ -69,9 +69,11 @@ typedef struct mts_params {
     int precision;   /* MTS_FP64 / MTS_FP32DEV                 default FP64  */
     int multiblock;  /* 0 single-block grid (the only mode in this build)   */
     int device;      /* HIP device ordinal                     default 0     */
-    int periodic_y;  /* 0: links crossing only the y-seam are dropped, as at
-                      * coreLB.hpp:301-331 (default); 1: they are periodic
-                      * links like the x-seam ones (true y-periodicity)      */
+    int periodic_y;  /* 0: the reference's seam rule (coreLB.hpp:301-331):
+                      *    x-seam links periodic (overwrite after the wall
+                      *    rule), y-seam-only links dropped (default);
+                      * 1: torus -- links across either seam are classified
+                      *    like interior links (mass-conserving)            */
 } mts_params;
 
 /* Fills the defaults above. */

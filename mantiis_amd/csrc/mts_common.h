@@ -26,11 +26,12 @@ constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 //   bits 16..23  WALL: the outgoing link (x, ic) is a boundary link;
 //   bits 24..29  TYPE: boundary type 0..51 (grid.hpp:1111-1157), 63 = none;
 //   bit  30      FLUID: the site is a cell;
-//   bit  31      SEAM: the site is in column 0 or nx-1 (or row 0 or ny-1),
-//                where an IN link whose source lies across the seam is a
-//                periodic link that overwrites the slot after the wall rule
-//                (coreLB.hpp:969-977).  With the reference's seam rule only
-//                x-seam links are periodic; y-only seam links carry no IN bit.
+//   bit  31      SEAM: the site is in column 0 or nx-1, where an IN link whose
+//                source lies across the x-seam is a periodic link that
+//                overwrites the slot after the wall rule (coreLB.hpp:969-977).
+//                Links crossing only the y-seam are dropped (no IN bit).
+// On a torus lattice (mts_params.periodic_y = 1) no site carries SEAM: links
+// across both seams are classified like interior links.
 constexpr uint32_t IN_SHIFT = 0, OUT_SHIFT = 8, WALL_SHIFT = 16, TYPE_SHIFT = 24;
 constexpr uint32_t TYPE_NONE = 63;
 constexpr uint32_t FLUID_BIT = 1u << 30;
@@ -61,6 +62,7 @@ void compute_physics(Physics& ph);
 // Host-side description of one single-block lattice.
 struct Geometry {
     int nx = 0, ny = 0;
+    bool torus = false;                  // seams classified like interior links
     int64_t stride = 0;                  // elements per population plane (>= nx*ny, multiple of 256)
     int64_t ncell = 0;                   // fluid cells
     std::vector<int32_t> cellPos;        // compact cell -> dense site y*nx + x
@@ -76,7 +78,7 @@ struct Geometry {
 // Builds the single-block grid exactly as Grid2D::initialize + LB2D::LB2D at
 // one rank.  Returns 0 or a negative MTS_ERR_* code with a message in err.
 int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
-                      const Physics& ph, bool periodic_y, Geometry& g, std::string& err);
+                      const Physics& ph, bool torus, Geometry& g, std::string& err);
 
 // Decodes the link lists of coreLB.hpp:272-366 from the link words.
 void decode_links(const Geometry& g, int which, std::vector<int64_t>& id, std::vector<int64_t>& loc,

@@ -71,7 +71,7 @@ void compute_physics(Physics& ph) {
 static int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
-                      const Physics& ph, bool periodic_y, Geometry& g, std::string& err) {
+                      const Physics& ph, bool torus, Geometry& g, std::string& err) {
     if (nx < 4 || ny < 4) {
         err = "lattice must be at least 4x4";
         return MTS_ERR_ARG;
@@ -79,6 +79,7 @@ int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, con
     g = Geometry();
     g.nx = nx;
     g.ny = ny;
+    g.torus = torus;
     const int64_t P = (int64_t)nx * ny;
     g.stride = (P + 255) / 256 * 256;
 
@@ -179,8 +180,7 @@ int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, con
         const int x = (int)(p % nx);
         const int t = g.bndTypes[c];
         uint32_t w = FLUID_BIT | ((uint32_t)(t < 0 ? TYPE_NONE : t) << TYPE_SHIFT);
-        const int y = (int)(p / nx);
-        if (x == 0 || x == nx - 1 || y == 0 || y == ny - 1) w |= SEAM_BIT;
+        if (!torus && (x == 0 || x == nx - 1)) w |= SEAM_BIT;
         g.meta[p] |= w;
     }
     for (int64_t c = 0; c < n; c++) {
@@ -194,13 +194,13 @@ int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, con
                 const int64_t q = g.cellPos[nb];
                 const int nxq = (int)(q % nx), nyq = (int)(q / nx);
                 const bool xp = std::abs(x - nxq) > 1, yp = std::abs(y - nyq) > 1;
-                if (!xp && !yp) {
+                if (torus || (!xp && !yp)) {
                     if (ifs) {  // stream link c -> nb
                         g.meta[q] |= 1u << (IN_SHIFT + ic - 1);
                         g.meta[p] |= 1u << (OUT_SHIFT + ic - 1);
                     } else
                         g.meta[p] |= 1u << (WALL_SHIFT + ic - 1);
-                } else if (xp || (periodic_y && yp)) {  // periodic link c -> nb (ifStream ignored)
+                } else if (xp) {  // periodic link c -> nb (ifStream ignored)
                     g.meta[q] |= 1u << (IN_SHIFT + ic - 1);
                     g.meta[p] |= 1u << (OUT_SHIFT + ic - 1);
                 }
@@ -242,7 +242,7 @@ void decode_links(const Geometry& g, int which, std::vector<int64_t>& id, std::v
             }
             const int xq = wrap(x + EX[ic], nx), yq = wrap(y + EY[ic], ny);
             const int64_t q = (int64_t)yq * nx + xq;
-            const bool xseam = (x + EX[ic] < 0) || (x + EX[ic] >= nx) || (y + EY[ic] < 0) || (y + EY[ic] >= ny);
+            const bool xseam = !g.torus && ((x + EX[ic] < 0) || (x + EX[ic] >= nx));
             const bool linked = cellOf[q] >= 0 && ((g.meta[q] >> (IN_SHIFT + ic - 1)) & 1u);
             if (which == 0 && linked && !xseam) { id.push_back(c); loc.push_back(cellOf[q]); icv.push_back(ic); }
             if (which == 2 && linked && xseam) { id.push_back(c); loc.push_back(cellOf[q]); icv.push_back(ic); }

@@ -84,10 +84,10 @@ struct Lat {
         ys = ys < 0 ? ys + ny : (ys >= ny ? ys - ny : ys);
         return (int64_t)ys * nx + xs;
     }
-    // slot j of site (x, y) is fed across a seam (a periodic link when its IN bit is set)
+    // slot j of site (x, y) is fed across the x-seam (a periodic link when its
+    // IN bit is set; only consulted for sites carrying SEAM_BIT)
     __device__ __forceinline__ bool seam_in(int x, int y, int j) const {
-        return (x == 0 && ex(j) == 1) || (x == nx - 1 && ex(j) == -1) || (y == 0 && ey(j) == 1) ||
-               (y == ny - 1 && ey(j) == -1);
+        return (x == 0 && ex(j) == 1) || (x == nx - 1 && ex(j) == -1);
     }
 };
 

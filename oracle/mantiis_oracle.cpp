@@ -357,8 +357,8 @@ struct Sim {
     double diff = 1.0;
     std::vector<double> diff_over_time;
     int collision = 2, wall = 2;
-    // false: the reference's seam rule (y-seam-only links dropped,
-    // coreLB.hpp:301-331); true: y-seam links periodic like x-seam links.
+    // false: the reference's seam rule (coreLB.hpp:301-331); true: torus, the
+    // links across both seams are classified like interior links.
     bool periodicY = false;
 
     std::vector<double> rho, rho_old, ux, uy, Fx, Fy, uxsq, uysq, usq, usq_old, usqMinusOld;
@@ -398,6 +398,7 @@ struct Sim {
                 if (c >= 0 && c < G.size) {
                     bool xp = std::abs(G.gridIJ[2 * i] - G.gridIJ[2 * c]) > seam;
                     bool yp = std::abs(G.gridIJ[2 * i + 1] - G.gridIJ[2 * c + 1]) > seam;
+                    if (periodicY) xp = yp = false;  // torus: seams like interior links
                     if (!xp && !yp) {
                         bool isStream = (bt == -1) || T.ifStream[bt][ic];
                         if (isStream) {
@@ -407,7 +408,7 @@ struct Sim {
                             boundaryID.push_back(i); boundaryIC.push_back(ic);
                             bfIC[i].push_back(ic);
                         }
-                    } else if (xp || (periodicY && yp)) {
+                    } else if (xp) {
                         periodicID.push_back(i); periodicI2.push_back(c); periodicIC.push_back(ic);
                         sfIC[i].push_back(ic); sfI2[i].push_back(c);
                     }

@@ -1,0 +1,68 @@
+"""Drivers written like the reference's test/transport_test.cpp and
+test/adsorption_test.cpp, compiled against this repository's include/ and
+run the way test/run_tests.sh runs them (1 thread, single-block, 64, 100
+steps); outputs compared with the gold files at the reference's criterion
+(max-abs 1e-10, scripts/compare_files.py:16-19)."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from _gpu_util import banner
+
+pytestmark = pytest.mark.gpu
+ROOT = O.ROOT
+
+
+def build(driver, tmp_path):
+    exe = tmp_path / driver
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I" + ROOT, "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "drivers", driver + ".cpp"), "-L" + os.path.join(ROOT, "mantiis_amd"),
+                    "-lmantiis_hip", "-Wl,-rpath," + os.path.join(ROOT, "mantiis_amd"), "-o", str(exe)], check=True)
+    return exe
+
+
+def fixture_copy(tmp_path):
+    d = tmp_path / "io"
+    d.mkdir()
+    for name in ("pore.dat", "Kn.dat", "localporesize.dat"):
+        shutil.copy(os.path.join(O.FIXTURES, name), d / name)
+    return str(d) + "/"
+
+
+@pytest.mark.parametrize("kind", ["transport", "adsorption"])
+def test_reference_style_driver_reproduces_gold(kind, tmp_path):
+    banner(f"drop-in {kind} driver 64x64 100 steps")
+    exe = build(kind + "_driver", tmp_path)
+    folder = fixture_copy(tmp_path)
+    r = subprocess.run([str(exe), "1", "False", "64", folder, "100"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-600:])
+    assert r.returncode == 0, r.stderr
+    for name in ("ux", "uy", "rho"):
+        a = np.loadtxt(folder + name + ".txt")
+        g = np.loadtxt(os.path.join(O.FIXTURES, "gold", kind, name + ".txt"))
+        err = np.abs(a - g).max()
+        print(name, err)
+        assert err <= 1e-10, name
+    conv = np.loadtxt(folder + "convergence.txt")
+    assert conv.size == 100
+    assert os.path.exists(folder + "f.txt") and os.path.exists(folder + "SimulationDetails.txt")
+    assert os.path.exists(folder + "ux_0.txt")  # dump at t = 0 (mantiisApp.hpp:328)
+
+
+def test_method_by_method_driver(tmp_path):
+    banner("drop-in LB2D method-by-method transport driver 64x64 100 steps")
+    exe = build("phase_driver", tmp_path)
+    folder = fixture_copy(tmp_path)
+    r = subprocess.run([str(exe), "64", folder, "100", folder + "m_"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:])
+    assert r.returncode == 0, r.stderr
+    for name in ("ux", "uy", "rho"):
+        a = np.loadtxt(folder + "m_" + name + ".txt")
+        g = np.loadtxt(os.path.join(O.FIXTURES, "gold", "transport", name + ".txt"))
+        assert np.abs(a - g).max() <= 1e-10, name
+    conv = np.loadtxt(folder + "m_convergence.txt")
+    assert abs(conv[0] - 0.466825156246162) < 1e-13 and abs(conv[99] - 0.00151596085691759) < 1e-14

@@ -118,8 +118,9 @@ def test_transport_disk_packing_fp64():
 
 
 def test_physics_mode_y_periodic_fp64():
-    """periodic_y=1: links across the y-seam are periodic (not dropped)."""
-    banner("transport physics mode (y-periodic) 128x128 FP64 100 steps")
+    """periodic_y=1 (torus): links across both seams are classified like
+    interior links; the library matches the oracle bit for bit and conserves mass."""
+    banner("transport torus 128x128 FP64 100 steps")
     pore = G.disk_packing(128, 0.6, 4, 10, seed=11)
     kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
     o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-6, periodic_y=True)
@@ -128,6 +129,7 @@ def test_physics_mode_y_periodic_fp64():
     for which in ("stream", "boundary", "periodic"):
         a, b = o.links(which), g.links(which)
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2]), which
+    assert len(g.links("periodic")[0]) == 0
     m0 = g.diagnostics()[0]
     o.run("transport", 100, 0.0)
     g.step("transport", 100)

@@ -140,3 +140,34 @@ def test_mass_conserved_with_bounce_back_and_periodic():
         m0 = o.get("f").sum()
         o.run("transport", 50, 0.0)
         assert abs(o.get("f").sum() - m0) / m0 < 1e-13
+
+
+def test_torus_mode_conserves_mass():
+    """periodic_y (torus): seams classified like interior links, no periodic
+    list, total mass invariant with SRBB and BB walls."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from mantiis_amd import geometry as G
+    pore = G.disk_packing(64, 0.6, 3, 8, seed=11)
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    for wall in (0, 2):
+        o = O.Oracle(pore, kn, lp, Fbody=1e-6, periodic_y=True)
+        o.set_operators(2, wall)
+        assert len(o.links("periodic")[0]) == 0
+        m0 = o.get("f").sum()
+        o.run("transport", 100, 0.0)
+        assert abs(o.get("f").sum() - m0) / m0 < 1e-13
+
+
+def test_reference_seam_rule_loses_mass_at_the_y_seam():
+    """With the reference's seam rule a fluid y-seam drops populations
+    (coreLB.hpp:301-331): mass decreases."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from mantiis_amd import geometry as G
+    pore = G.disk_packing(64, 0.6, 3, 8, seed=11)
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    o = O.Oracle(pore, kn, lp, Fbody=1e-6)
+    m0 = o.get("f").sum()
+    o.run("transport", 20, 0.0)
+    assert o.get("f").sum() < m0 * (1 - 1e-4)
