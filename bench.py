@@ -138,7 +138,7 @@ def main():
         a.fill_(1.0)
         torch.cuda.synchronize()
         for _ in range(5):
-            b.copy_(a)
+            torch.mul(a, 1.0, out=b)  # an elementwise shader kernel: reads 1 GiB, writes 1 GiB
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "launches": 5}))
         return 0

@@ -67,7 +67,7 @@ typedef struct mts_params {
     int collision;   /* MTS_COLL_*                             default RMRT  */
     int wall;        /* MTS_WALL_*                             default SRBB  */
     int precision;   /* MTS_FP64 / MTS_FP32DEV                 default FP64  */
-    int multiblock;  /* 0 single-block grid (the only mode in this build)   */
+    int multiblock;  /* 1: two-level quadtree grid (grid.hpp:313-399)      */
     int device;      /* HIP device ordinal                     default 0     */
     int periodic_y;  /* 0: the reference's seam rule (coreLB.hpp:301-331):
                       *    x-seam links periodic (overwrite after the wall
@@ -99,6 +99,27 @@ int mts_load_existing_model(const char* folder, int n, int32_t* pore, double* kn
 int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
                const mts_params* p, mts_sim* out);
 int mts_destroy(mts_sim s);
+
+/* ---- Grid2D on the host (grid.hpp:167-1157), no device needed.
+ * mts_grid_create builds the cell list of a row-major [y][x] geometry exactly
+ * as Grid2D(shape) + Grid2D::initialize (single-block, or the two-level
+ * quadtree when p->multiblock is set; p->periodic_y selects the seam rule).
+ * Fails with MTS_ERR_GEOMETRY on an unsupported wall pattern. */
+typedef struct mts_grid_s* mts_grid;
+int mts_grid_create(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
+                    const mts_params* p, mts_grid* out);
+int mts_grid_destroy(mts_grid g);
+/* *n = Grid2D::globalGridSize, *max_level = Grid2D::maxLevel */
+int mts_grid_size(mts_grid g, int64_t* n, int32_t* max_level);
+/* Grid2D arrays (any pointer may be NULL): gridID [n], gridIJ [n*2] = {x, y},
+ * gridLevel [n], gridConnect [n*9], bndTypes [n], gridType [n],
+ * gridIsBuffer [n], taus [n] (relaxation time of each cell). */
+int mts_grid_arrays(mts_grid g, int64_t* gridID, int32_t* gridIJ, int32_t* level, int64_t* connect, int32_t* bnd,
+                    int32_t* type, int32_t* isBuffer, double* taus);
+/* which = 0: Grid2D::gridParentChild rows {cell, tl, tr, bl, br};
+ * 1: coarseBuffer[21] rows; 2: fineBuffer[12] rows.  Rows of 5 int64;
+ * call with rows = NULL to get *count (rows). */
+int mts_grid_rows(mts_grid g, int which, int64_t* rows, int64_t* count);
 
 /* Grid2D::globalGridSize (grid.hpp:195). */
 int mts_cell_count(mts_sim s, int64_t* n);

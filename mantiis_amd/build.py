@@ -34,9 +34,10 @@ def build_lib(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     subprocess.run([sys.executable, os.path.join(CSRC, "gen_tables.py")], check=True)
     objs = []
-    o = os.path.join(BUILD, "mts_grid.o")
-    _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "mts_grid.cpp"), "-o", o], verbose)
-    objs.append(o)
+    for src in ("mts_grid.cpp", "mts_multigrid.cpp"):
+        o = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", o], verbose)
+        objs.append(o)
     for src in ("mts_kernels.hip", "mts_api.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o], verbose)

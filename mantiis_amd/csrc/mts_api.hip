@@ -14,6 +14,7 @@
 #include "mantiis_hip.h"
 #include "mts_common.h"
 #include "mts_kernels.h"
+#include "mts_multigrid.h"
 
 namespace mts {
 void upload_xynorm(hipStream_t st);
@@ -264,6 +265,106 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
 
 int mts_destroy(mts_sim s) {
     delete s;
+    return MTS_OK;
+}
+
+}  // extern "C"
+
+// Host grid handle: the single-block Geometry or the quadtree MultiGeometry
+// presented as one cell list.
+struct mts_grid_s {
+    bool multi = false;
+    Geometry single;
+    MultiGeometry mg;
+    int64_t n() const { return multi ? mg.ncell : single.ncell; }
+};
+
+static int build_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore, const mts_params& prm,
+                      const Physics& ph, mts_grid_s& g, std::string& err) {
+    g.multi = prm.multiblock != 0;
+    if (g.multi) return build_multi_grid(nx, ny, pore, kn, locpore, ph, prm.periodic_y != 0, g.mg, err);
+    return build_single_grid(nx, ny, pore, kn, locpore, ph, prm.periodic_y != 0, g.single, err);
+}
+
+extern "C" {
+
+int mts_grid_create(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore, const mts_params* p,
+                    mts_grid* out) {
+    if (!out || !pore || !kn || !locpore) return fail(MTS_ERR_ARG, "null argument");
+    *out = nullptr;
+    mts_params prm;
+    if (p) prm = *p;
+    else mts_default_params(&prm);
+    Physics ph;
+    ph.Cl = prm.dx;
+    ph.Fbody = prm.Fbody;
+    ph.T_phy = prm.T;
+    ph.P_phy = prm.P;
+    compute_physics(ph);
+    if (!ph.cubic_ok) return fail(MTS_ERR_PHYSICS, "PR-EOS: no single positive real root at this (T, P)");
+    mts_grid_s* g = new (std::nothrow) mts_grid_s();
+    if (!g) return fail(MTS_ERR_OOM, "host allocation");
+    std::string err;
+    const int rc = build_grid(nx, ny, pore, kn, locpore, prm, ph, *g, err);
+    if (rc) {
+        delete g;
+        return fail(rc, err);
+    }
+    *out = g;
+    return MTS_OK;
+}
+
+int mts_grid_destroy(mts_grid g) {
+    delete g;
+    return MTS_OK;
+}
+
+int mts_grid_size(mts_grid g, int64_t* n, int32_t* max_level) {
+    if (!g) return fail(MTS_ERR_ARG, "null grid");
+    if (n) *n = g->n();
+    if (max_level) *max_level = g->multi ? g->mg.maxLevel : 1;
+    return MTS_OK;
+}
+
+int mts_grid_arrays(mts_grid g, int64_t* gridID, int32_t* gridIJ, int32_t* level, int64_t* connect, int32_t* bnd,
+                    int32_t* type, int32_t* isBuffer, double* taus) {
+    if (!g) return fail(MTS_ERR_ARG, "null grid");
+    const int64_t n = g->n();
+    for (int64_t c = 0; c < n; c++) {
+        if (g->multi) {
+            const MultiGeometry& m = g->mg;
+            if (gridID) gridID[c] = m.gridID[c];
+            if (gridIJ) { gridIJ[2 * c] = m.cx[c]; gridIJ[2 * c + 1] = m.cy[c]; }
+            if (level) level[c] = m.level[c];
+            if (bnd) bnd[c] = m.bnd[c];
+            if (type) type[c] = m.type[c];
+            if (isBuffer) isBuffer[c] = m.isBuffer[c];
+            if (taus) taus[c] = m.taus[c];
+            if (connect)
+                for (int k = 0; k < 9; k++) connect[9 * c + k] = m.connect[9 * c + k];
+        } else {
+            const Geometry& s = g->single;
+            const int64_t p = s.cellPos[c];
+            if (gridID) gridID[c] = (p % s.nx) + (p / s.nx) * (int64_t)s.ny;
+            if (gridIJ) { gridIJ[2 * c] = (int32_t)(p % s.nx); gridIJ[2 * c + 1] = (int32_t)(p / s.nx); }
+            if (level) level[c] = 1;
+            if (bnd) bnd[c] = s.bndTypes[c];
+            if (type) type[c] = 0;
+            if (isBuffer) isBuffer[c] = 0;
+            if (taus) taus[c] = s.taus[c];
+            if (connect)
+                for (int k = 0; k < 9; k++) connect[9 * c + k] = s.gridConnect[9 * c + k];
+        }
+    }
+    return MTS_OK;
+}
+
+int mts_grid_rows(mts_grid g, int which, int64_t* rows, int64_t* count) {
+    if (!g || !count || which < 0 || which > 2) return fail(MTS_ERR_ARG, "bad argument");
+    static const std::vector<int64_t> none;
+    const std::vector<int64_t>& v = !g->multi ? none : which == 0 ? g->mg.parentChild : which == 1 ? g->mg.coarseRows : g->mg.fineRows;
+    *count = (int64_t)v.size() / 5;
+    if (rows) std::copy(v.begin(), v.end(), rows);
     return MTS_OK;
 }
 

@@ -1,0 +1,372 @@
+// Host-side set-up of a two-level quadtree lattice (the multi-block grid of
+// grid.hpp:313-1157): wall distance, coarse/fine cell selection, buffer
+// cells, connectivity, parent/child and interface tables, and the per-level
+// site images with link words that the HIP kernels decode.  Plain C++.
+//
+// Coarse cells are the aligned 2x2 fluid blocks whose four pixels are at a
+// Euclidean distance >= 6 from every wall pixel (the quadtree split rule of
+// grid.hpp:1180-1214 with maxAllowedLevel = 2).  Cells are numbered in
+// row-major order of their top-left pixel, followed by the buffer cells in
+// creation order (grid.hpp:808-888).  On a torus (periodic_y = 1) the
+// distance transform and the buffer rule wrap around both seams; otherwise
+// they stop at the image edge as the reference's do.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+
+#include "mantiis_hip.h"
+#include "mts_common.h"
+#include "mts_multigrid.h"
+#include "mts_tables.inc"
+
+namespace mts {
+
+namespace {
+
+// 1-D squared distance transform (lower envelope of parabolas).
+void dt1d(const std::vector<double>& f, std::vector<double>& d) {
+    const int n = (int)f.size();
+    std::vector<int> v(n);
+    std::vector<double> z(n + 1);
+    int k = 0;
+    v[0] = 0;
+    z[0] = -1e300;
+    z[1] = 1e300;
+    for (int q = 1; q < n; q++) {
+        if (f[q] >= 1e299) continue;
+        if (f[v[k]] >= 1e299) {
+            v[k] = q;
+            continue;
+        }
+        double s;
+        while (true) {
+            s = ((f[q] + (double)q * q) - (f[v[k]] + (double)v[k] * v[k])) / (2.0 * q - 2.0 * v[k]);
+            if (s <= z[k] && k > 0) k--;
+            else break;
+        }
+        if (s <= z[k]) {  // k == 0
+            v[k] = q;
+            z[k + 1] = 1e300;
+            continue;
+        }
+        k++;
+        v[k] = q;
+        z[k] = s;
+        z[k + 1] = 1e300;
+    }
+    k = 0;
+    for (int q = 0; q < n; q++) {
+        while (z[k + 1] < q) k++;
+        const double dq = (double)(q - v[k]);
+        d[q] = f[v[k]] >= 1e299 ? 1e300 : dq * dq + f[v[k]];
+    }
+}
+
+// Exact squared Euclidean distance of every pixel to the nearest wall pixel
+// (Felzenszwalb-Huttenlocher separable transform), optionally periodic.
+std::vector<double> edt2(int nx, int ny, const int32_t* pore, bool wrap) {
+    const double INF = 1e300;
+    std::vector<double> g((size_t)nx * ny);
+    const int rep = wrap ? 3 : 1;
+    {
+        std::vector<double> f((size_t)ny * rep), d(f.size());
+        for (int x = 0; x < nx; x++) {
+            for (int r = 0; r < rep; r++)
+                for (int y = 0; y < ny; y++) f[(size_t)r * ny + y] = pore[(size_t)y * nx + x] == 0 ? 0.0 : INF;
+            dt1d(f, d);
+            for (int y = 0; y < ny; y++) g[(size_t)y * nx + x] = d[(size_t)(wrap ? ny : 0) + y];
+        }
+    }
+    std::vector<double> out((size_t)nx * ny);
+    std::vector<double> f((size_t)nx * rep), d(f.size());
+    for (int y = 0; y < ny; y++) {
+        for (int r = 0; r < rep; r++)
+            for (int x = 0; x < nx; x++) f[(size_t)r * nx + x] = g[(size_t)y * nx + x];
+        dt1d(f, d);
+        for (int x = 0; x < nx; x++) out[(size_t)y * nx + x] = d[(size_t)(wrap ? nx : 0) + x];
+    }
+    return out;
+}
+
+int wrapc(int v, int n) { return ((v % n) + n) % n; }
+
+}  // namespace
+
+int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore, const Physics& ph,
+                     bool torus, MultiGeometry& G, std::string& err) {
+    G = MultiGeometry();
+    if (nx != ny || nx < 8 || (nx & (nx - 1)) != 0) {
+        err = "Matrix dimensions must be divisible by 2 in each direction. (multi-block lattices are square, "
+              "power-of-two sided)";
+        return MTS_ERR_ARG;
+    }
+    for (int64_t p = 0; p < (int64_t)nx * ny; p++)
+        if (pore[p] != 0 && pore[p] != 1) {
+            err = "pore image must hold only 0 (solid) and 1 (fluid)";
+            return MTS_ERR_ARG;
+        }
+    G.nx = nx;
+    G.ny = ny;
+    G.torus = torus;
+    // ---- level decision: aligned 2x2 fluid block, wall distance >= 6
+    const std::vector<double> d2 = edt2(nx, ny, pore, torus);
+    auto dist = [&](int x, int y) -> double {
+        const double v = d2[(size_t)y * nx + x];
+        return v >= 1e299 ? 1e30 : (double)(float)std::sqrt(v);
+    };
+    const int cnx = nx / 2, cny = ny / 2;
+    std::vector<uint8_t> coarse((size_t)cnx * cny, 0);
+    bool any = false;
+    for (int by = 0; by < cny; by++)
+        for (int bx = 0; bx < cnx; bx++) {
+            bool ok = true;
+            for (int dy = 0; dy < 2 && ok; dy++)
+                for (int dx = 0; dx < 2 && ok; dx++) {
+                    const int x = 2 * bx + dx, y = 2 * by + dy;
+                    ok = pore[(size_t)y * nx + x] == 1 && dist(x, y) >= 6.0;
+                }
+            coarse[(size_t)by * cnx + bx] = ok;
+            any = any || ok;
+        }
+    G.maxLevel = any ? 2 : 1;
+    const int L = G.maxLevel;
+    auto gid = [&](int x, int y, int lvl) -> int64_t { return (int64_t)x + (int64_t)y * ny + (int64_t)(lvl - 1) * ny * nx; };
+
+    // ---- original cells (incl. wall pixels) in row-major order of top-left pixel
+    struct Orig {
+        int x, y, lvl;
+        bool fluid;
+    };
+    std::vector<Orig> orig;
+    std::vector<int32_t> origAt((size_t)nx * ny, -1);  // top-left pixel -> orig index
+    for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+            const bool inCoarse = L > 1 && coarse[(size_t)(y / 2) * cnx + x / 2];
+            if (inCoarse) {
+                if ((x & 1) == 0 && (y & 1) == 0) {
+                    origAt[(size_t)y * nx + x] = (int32_t)orig.size();
+                    orig.push_back({x, y, 2, true});
+                }
+            } else {
+                origAt[(size_t)y * nx + x] = (int32_t)orig.size();
+                orig.push_back({x, y, 1, pore[(size_t)y * nx + x] == 1});
+            }
+        }
+    // existence of a cell (original or buffer) per level position
+    std::vector<uint8_t> exist1((size_t)nx * ny, 0), exist2((size_t)cnx * cny, 0);
+    for (const Orig& o : orig) {
+        if (o.lvl == 1) exist1[(size_t)o.y * nx + o.x] = 1;
+        else exist2[(size_t)(o.y / 2) * cnx + o.x / 2] = 1;
+    }
+    // ---- buffer cells (grid.hpp:665-806), 2 levels
+    struct Buf {
+        int x, y, lvl, type;
+    };
+    std::vector<Buf> bufs;
+    if (L > 1) {
+        for (const Orig& o : orig) {
+            if (o.lvl != 2) continue;
+            // 3x3 level-2 neighbourhood, column-major (x outer, y inner)
+            std::vector<std::pair<int, int>> nm;   // existing level-2 neighbours (top-left pixel)
+            std::vector<std::pair<int, int>> mk;   // missing positions (pixel coords, may be off-image on a torus: wrapped)
+            for (int ox = -2; ox <= 2; ox += 2)
+                for (int oy = -2; oy <= 2; oy += 2) {
+                    int x = o.x + ox, y = o.y + oy;
+                    if (!torus && (x < 0 || y < 0 || x >= nx || y >= ny)) continue;  // "-1" entries
+                    const int xw = wrapc(x, nx), yw = wrapc(y, ny);
+                    if (exist2[(size_t)(yw / 2) * cnx + xw / 2] &&
+                        // an original coarse cell (buffers are added later)
+                        origAt[(size_t)yw * nx + xw] >= 0 && orig[origAt[(size_t)yw * nx + xw]].lvl == 2)
+                        nm.push_back({x, y});
+                    else
+                        mk.push_back({x, y});
+                }
+            if (mk.empty()) continue;
+            // cells whose top-left pixel lies within +-2 of this cell's, in cell order
+            std::vector<std::pair<int, std::pair<int, int>>> inr;  // (orig index, relative position)
+            for (int oy = -2; oy <= 2; oy++)
+                for (int ox = -2; ox <= 2; ox++) {
+                    int x = o.x + ox, y = o.y + oy;
+                    if (!torus && (x < 0 || y < 0 || x >= nx || y >= ny)) continue;
+                    const int k = origAt[(size_t)wrapc(y, ny) * nx + wrapc(x, nx)];
+                    if (k >= 0) inr.push_back({k, {x, y}});
+                }
+            if (torus) {
+                // a cell may be reached twice through the wrap on tiny lattices; keep the first
+                std::sort(inr.begin(), inr.end(), [&](const auto& a, const auto& b) {
+                    return a.first != b.first ? a.first < b.first : a.second < b.second;
+                });
+                inr.erase(std::unique(inr.begin(), inr.end(), [](const auto& a, const auto& b) { return a.first == b.first; }),
+                          inr.end());
+            } else {
+                std::sort(inr.begin(), inr.end());
+            }
+            std::vector<std::pair<int, int>> missing = mk;
+            std::vector<bool> used(missing.size(), false);
+            for (const auto& e : inr) {
+                const Orig& c = orig[e.first];
+                const int rx = e.second.first, ry = e.second.second;
+                bool isNb = false;
+                if (c.lvl == 2)
+                    for (const auto& q : nm)
+                        if (wrapc(q.first, nx) == c.x && wrapc(q.second, ny) == c.y) isNb = true;
+                if (isNb) {
+                    // fine buffer children of this coarse cell (type 12)
+                    const int cand[4][2] = {{o.x, o.y}, {o.x + 1, o.y}, {o.x, o.y + 1}, {o.x + 1, o.y + 1}};
+                    for (auto& cd : cand)
+                        if (!exist1[(size_t)cd[1] * nx + cd[0]]) {
+                            exist1[(size_t)cd[1] * nx + cd[0]] = 1;
+                            bufs.push_back({cd[0], cd[1], 1, 12});
+                        }
+                } else {
+                    // coarse buffer at the missing position closest to this cell (type 21)
+                    int best = 0;
+                    double minD = std::numeric_limits<double>::max();
+                    for (size_t m = 0; m < missing.size(); m++) {
+                        float mx, my;
+                        if (used[m]) {
+                            // the reference marks a consumed key -1, whose position
+                            // ind2sub(-1) reads as (-1, 0); on a torus it is skipped
+                            if (torus) continue;
+                            mx = -1.0f;
+                            my = 0.0f;
+                        } else {
+                            mx = (float)missing[m].first;
+                            my = (float)missing[m].second;
+                        }
+                        const float ddx = ((float)rx + 0.5f) - (mx + 0.5f), ddy = ((float)ry + 0.5f) - (my + 0.5f);
+                        const double dd = std::sqrt((double)(std::pow(ddx, 2) + std::pow(ddy, 2)));
+                        if (dd < minD) {
+                            minD = dd;
+                            best = (int)m;
+                        }
+                    }
+                    if (!used[best]) {
+                        const int mx = wrapc(missing[best].first, nx), my = wrapc(missing[best].second, ny);
+                        if (!exist2[(size_t)(my / 2) * cnx + mx / 2]) {
+                            exist2[(size_t)(my / 2) * cnx + mx / 2] = 1;
+                            bufs.push_back({mx, my, 2, 21});
+                            used[best] = true;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // ---- grid arrays (fluid original cells, then buffers)
+    for (const Orig& o : orig)
+        if (o.fluid) {
+            G.cx.push_back(o.x);
+            G.cy.push_back(o.y);
+            G.level.push_back(o.lvl);
+            G.type.push_back(0);
+            G.isBuffer.push_back(0);
+        }
+    for (const Buf& b : bufs) {
+        G.cx.push_back(b.x);
+        G.cy.push_back(b.y);
+        G.level.push_back(b.lvl);
+        G.type.push_back(b.type);
+        G.isBuffer.push_back(1);
+    }
+    const int64_t n = (int64_t)G.cx.size();
+    G.ncell = n;
+    G.gridID.resize(n);
+    for (int64_t c = 0; c < n; c++) G.gridID[c] = gid(G.cx[c], G.cy[c], G.level[c]);
+    // per-level index images: cell at a level position
+    std::vector<int64_t> at1((size_t)nx * ny, -1), at2((size_t)cnx * cny, -1);
+    std::vector<uint8_t> orig1((size_t)nx * ny, 0), orig2((size_t)cnx * cny, 0);
+    for (int64_t c = n - 1; c >= 0; c--) {
+        if (G.level[c] == 1) at1[(size_t)G.cy[c] * nx + G.cx[c]] = c;
+        else at2[(size_t)(G.cy[c] / 2) * cnx + G.cx[c] / 2] = c;
+    }
+    for (const Orig& o : orig) {  // positions of linInd (incl. wall pixels) per level
+        if (o.lvl == 1) orig1[(size_t)o.y * nx + o.x] = 1;
+        else orig2[(size_t)(o.y / 2) * cnx + o.x / 2] = 1;
+    }
+    // ---- connectivity (grid.hpp:916-972)
+    G.connect.assign(n * 9, -2);
+    for (int64_t c = 0; c < n; c++) {
+        const int jump = G.level[c] == 1 ? 1 : 2;
+        for (int k = 0; k < 9; k++) {
+            const int x = wrapc(G.cx[c] + EX[k] * jump, nx), y = wrapc(G.cy[c] + EY[k] * jump, ny);
+            if (G.level[c] == 1) {
+                const int64_t q = at1[(size_t)y * nx + x];
+                G.connect[c * 9 + k] = q >= 0 ? q : (orig1[(size_t)y * nx + x] ? -1 : -2);
+            } else {
+                const int64_t q = at2[(size_t)(y / 2) * cnx + x / 2];
+                G.connect[c * 9 + k] = q >= 0 ? q : (orig2[(size_t)(y / 2) * cnx + x / 2] ? -1 : -2);
+            }
+        }
+    }
+    // ---- boundary types of non-buffer cells (grid.hpp:1111-1157)
+    static const int order[9] = {7, 3, 6, 4, 0, 2, 8, 1, 5};
+    int codeToType[512];
+    for (int i = 0; i < 512; i++) codeToType[i] = -2;
+    for (int t = 0; t < MTS_NTYPES; t++) codeToType[MTS_BND_CODE[t]] = t;
+    codeToType[511] = -1;
+    G.bnd.assign(n, -1);
+    for (int64_t c = 0; c < n; c++) {
+        if (G.isBuffer[c]) continue;
+        int code = 0;
+        for (int b = 0; b < 9; b++) code = (code << 1) | (G.connect[c * 9 + order[b]] == -1 ? 0 : 1);
+        const int t = codeToType[code];
+        if (t == -2) {
+            err = "[ERROR]: Boundary type is not contained. (cell x=" + std::to_string(G.cx[c]) + " y=" +
+                  std::to_string(G.cy[c]) + ", code " + std::to_string(code) + ")";
+            return MTS_ERR_GEOMETRY;
+        }
+        G.bnd[c] = t;
+    }
+    // ---- parent/child (grid.hpp:974-1038) and buffer rows (grid.hpp:1041-1108)
+    std::vector<int64_t> rowOf(n, -1);
+    for (int64_t c = 0; c < n; c++) {
+        if (G.level[c] <= 1) continue;
+        rowOf[c] = (int64_t)G.parentChild.size() / 5;
+        G.parentChild.push_back(c);
+        const int sub[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+        for (auto& s : sub) G.parentChild.push_back(at1[(size_t)(G.cy[c] + s[1]) * nx + G.cx[c] + s[0]]);
+    }
+    for (int64_t c = 0; c < n; c++)
+        if (G.type[c] == 21) {
+            const int64_t r = rowOf[c];
+            G.coarseRows.insert(G.coarseRows.end(), G.parentChild.begin() + 5 * r, G.parentChild.begin() + 5 * r + 5);
+        }
+    {
+        std::vector<int64_t> ids;
+        for (int64_t c = 0; c < n; c++)
+            if (G.type[c] == 12) ids.push_back(c);
+        std::unordered_map<int64_t, int64_t> firstRow;
+        const int64_t rows = (int64_t)G.parentChild.size() / 5;
+        for (int64_t r = rows - 1; r >= 0; r--)
+            for (int k = 4; k >= 1; k--)
+                if (G.parentChild[5 * r + k] >= 0) firstRow[G.parentChild[5 * r + k]] = r;
+        for (size_t g4 = 0; g4 + 3 < ids.size(); g4 += 4) {
+            int64_t best = std::numeric_limits<int64_t>::max();
+            for (int k = 0; k < 4; k++) {
+                auto it = firstRow.find(ids[g4 + k]);
+                if (it != firstRow.end()) best = std::min(best, it->second);
+            }
+            if (best == std::numeric_limits<int64_t>::max()) {
+                err = "fine buffer cells without a parent";
+                return MTS_ERR_GEOMETRY;
+            }
+            const int64_t row[5] = {G.parentChild[5 * best], ids[g4], ids[g4 + 1], ids[g4 + 2], ids[g4 + 3]};
+            G.fineRows.insert(G.fineRows.end(), row, row + 5);
+        }
+    }
+    // ---- per-cell relaxation (per level, from the loaded maps)
+    G.taus.resize(n);
+    G.kn.resize(n);
+    for (int64_t c = 0; c < n; c++) {
+        const size_t p = (size_t)G.cy[c] * nx + G.cx[c];
+        const double K = kn[p], Lp = locpore[p] / std::pow(2.0, G.level[c] - 1);
+        G.taus[c] = 0.5 + std::sqrt(6.0 / M_PI) * Lp * K / (1 + 2 * K);
+        G.kn[c] = K;
+    }
+    return MTS_OK;
+}
+
+}  // namespace mts

@@ -30,7 +30,8 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_destroy", "mts_cell_count", "mts_grid_info", "mts_links", "mts_set_operators",
            "mts_set_body_force", "mts_initialize", "mts_step_transport", "mts_step_adsorption",
            "mts_step_coupled", "mts_residual", "mts_method", "mts_field_size", "mts_get_field",
-           "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_get_stream", "mts_synchronize"]
+           "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_get_stream", "mts_synchronize",
+           "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows"]
 
 
 class HipLibraryMissing(RuntimeError):
@@ -90,6 +91,11 @@ def load_library(path=LIB_PATH):
     lib.mts_diagnostics.argtypes = [C.c_void_p, _D]
     lib.mts_get_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
     lib.mts_synchronize.argtypes = [C.c_void_p]
+    lib.mts_grid_create.argtypes = [C.c_int, C.c_int, _I32, _D, _D, C.POINTER(Params), C.POINTER(C.c_void_p)]
+    lib.mts_grid_destroy.argtypes = [C.c_void_p]
+    lib.mts_grid_size.argtypes = [C.c_void_p, _I64, _I32]
+    lib.mts_grid_arrays.argtypes = [C.c_void_p, _I64, _I32, _I32, _I64, _I32, _I32, _I32, _D]
+    lib.mts_grid_rows.argtypes = [C.c_void_p, C.c_int, _I64, _I64]
     _LIB = lib
     return lib
 
@@ -243,3 +249,53 @@ class Lattice:
 
     def synchronize(self):
         _check(self.lib, self.lib.mts_synchronize(self.h))
+
+
+class HostGrid:
+    """Grid2D built on the host by the library (no device)."""
+
+    def __init__(self, pore, kn, locpore, **params):
+        self.lib = load_library()
+        pore = np.ascontiguousarray(pore, dtype=np.int32)
+        kn = np.ascontiguousarray(kn, dtype=np.float64)
+        locpore = np.ascontiguousarray(locpore, dtype=np.float64)
+        ny, nx = pore.shape
+        p = default_params(**params)
+        h = C.c_void_p()
+        _check(self.lib, self.lib.mts_grid_create(nx, ny, _p(pore, _I32), _p(kn, _D), _p(locpore, _D), C.byref(p),
+                                                  C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h.value:
+                self.lib.mts_grid_destroy(self.h)
+        except Exception:
+            pass
+
+    def size(self):
+        n, L = C.c_int64(), C.c_int32()
+        _check(self.lib, self.lib.mts_grid_size(self.h, C.byref(n), C.byref(L)))
+        return n.value, L.value
+
+    def arrays(self):
+        n, _ = self.size()
+        gid = np.zeros(n, np.int64)
+        ij = np.zeros((n, 2), np.int32)
+        lvl = np.zeros(n, np.int32)
+        con = np.zeros((n, 9), np.int64)
+        bnd = np.zeros(n, np.int32)
+        typ = np.zeros(n, np.int32)
+        isb = np.zeros(n, np.int32)
+        taus = np.zeros(n)
+        _check(self.lib, self.lib.mts_grid_arrays(self.h, _p(gid, _I64), _p(ij, _I32), _p(lvl, _I32), _p(con, _I64),
+                                                  _p(bnd, _I32), _p(typ, _I32), _p(isb, _I32), _p(taus, _D)))
+        return dict(gridID=gid, gridIJ=ij, level=lvl, connect=con, bnd=bnd, type=typ, isBuffer=isb, taus=taus)
+
+    def rows(self, which):
+        w = {"parent_child": 0, "coarse": 1, "fine": 2}[which]
+        cnt = C.c_int64()
+        _check(self.lib, self.lib.mts_grid_rows(self.h, w, None, C.byref(cnt)))
+        out = np.zeros((cnt.value, 5), np.int64)
+        _check(self.lib, self.lib.mts_grid_rows(self.h, w, _p(out, _I64), C.byref(cnt)))
+        return out

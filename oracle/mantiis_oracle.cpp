@@ -223,7 +223,7 @@ struct Grid {
     std::vector<int64_t> solidID;
     std::vector<double> locpore, kn;
     std::vector<int> bndTypes;
-    std::vector<int64_t> parentChild;          // [k*5+...]
+    std::vector<int64_t> parentChild;          // rows of {cell, tl, tr, bl, br}
     std::map<int, std::vector<std::vector<int64_t>>> coarseBuffer, fineBuffer;
     std::vector<int64_t> reconstructed;        // ny*nx, reconstructed.dat
 };
@@ -252,10 +252,12 @@ static void processSinglegrid(const Shape& S, Grid& G, std::vector<int64_t>& img
 
 // Declared here, defined in oracle/multiblock_oracle.cpp (grid.hpp:313-399,
 // 427-889, 974-1108, 1160-1269).
-void processMultigrid(const Shape& S, Grid& G, std::vector<int64_t>& img);
-void rebuildGridMulti(Grid& G, const std::vector<int64_t>& img, std::vector<int64_t>& linInd);
+void processMultigrid(const Shape& S, Grid& G, std::vector<int64_t>& img, bool torus);
+void rebuildGridMulti(Grid& G, const std::vector<int64_t>& img, std::vector<int64_t>& linInd, bool torus);
 void determineParentChild(Grid& G);
 void orderBufferIndex(Grid& G);
+struct Sim;
+void multigridTaus(Sim& s);
 
 // grid.hpp:427-889 for maxLevel == 1 (every unique image value is one pixel;
 // no interfaces so no buffers).  Cells are the pore pixels in row-major order.
@@ -418,7 +420,10 @@ struct Sim {
                         boundaryID.push_back(i); boundaryIC.push_back(ic);
                         bfIC[i].push_back(ic);
                     }
-                } else if (c == -1 && !T.ifStream[bt][ic]) {
+                } else if (c == -1 && (bt < 0 || !T.ifStream[bt][ic])) {
+                    // bt < 0 only for a buffer cell next to a wall across a
+                    // periodic seam, where the reference reads ifStream[-1]:
+                    // defined here as a wall link
                     boundaryID.push_back(i); boundaryIC.push_back(ic);
                     bfIC[i].push_back(ic);
                 }
@@ -564,7 +569,9 @@ struct Sim {
         for (size_t i = 0; i < boundaryID.size(); i++)
             if (G.gridLevel[boundaryID[i]] == lvl) {
                 double r = 0.4987 * std::pow(G.kn[boundaryID[i]], -0.131) - 0.25;
-                int srOpp = T.icsr[G.bndTypes[boundaryID[i]]][boundaryIC[i]];
+                const int bt = G.bndTypes[boundaryID[i]];
+                // typeless wall link of a buffer cell: full bounce-back target
+                int srOpp = bt < 0 ? OPP[boundaryIC[i]] : T.icsr[bt][boundaryIC[i]];
                 f[boundaryID[i] * NC + OPP[boundaryIC[i]]] += r * ftemp[boundaryID[i] * NC + boundaryIC[i]];
                 f[boundaryID[i] * NC + srOpp] += (1 - r) * ftemp[boundaryID[i] * NC + boundaryIC[i]];
             }
@@ -575,6 +582,10 @@ struct Sim {
             if (G.gridLevel[boundaryID[i]] != lvl) continue;
             int64_t k1 = boundaryID[i];
             int k2 = boundaryIC[i];
+            if (G.bndTypes[k1] < 0) {  // typeless wall link: plain bounce-back
+                f[k1 * NC + OPP[k2]] = ftemp[k1 * NC + k2];
+                continue;
+            }
             double Kdeno = 0.0, Kno = 0.0;
             double nx = T.xy_norm[G.bndTypes[k1]][0], ny = T.xy_norm[G.bndTypes[k1]][1];
             for (int ic = 0; ic < NC; ic++) {
@@ -867,8 +878,8 @@ void* orc_create(int N, const int* pore, const double* kn, const double* locpore
         s->G.nx = N; s->G.ny = N; s->G.isMultigrid = multiblock != 0;
         std::vector<int64_t> img, linInd;
         if (multiblock) {
-            orc::processMultigrid(s->S, s->G, img);
-            orc::rebuildGridMulti(s->G, img, linInd);
+            orc::processMultigrid(s->S, s->G, img, s->periodicY);
+            orc::rebuildGridMulti(s->G, img, linInd, s->periodicY);
         } else {
             orc::processSinglegrid(s->S, s->G, img);
             orc::rebuildGridSingle(s->G, img, linInd);
@@ -882,7 +893,7 @@ void* orc_create(int N, const int* pore, const double* kn, const double* locpore
         }
         orc::getBoundaryTypes(s->G, s->T);
         if (!multiblock) s->singleGridTaus();
-        else s->taus.assign(s->G.size, 0.0);  // set by orc_set_taus (per-level, OpenCV path)
+        else orc::multigridTaus(*s);
         s->buildLinks();
         s->initialize();
         return s;
@@ -1067,6 +1078,33 @@ void orc_complete_domain(void* h, double* ux, double* uy, double* rho) {
     static_cast<Sim*>(h)->completeDomain(ux, uy, rho);
 }
 int orc_psi_negative(void* h) { return static_cast<Sim*>(h)->psiNegative; }
+
+// Multi-block field on the pixel image: each pixel takes the value of the
+// non-buffer cell covering it (the assignment of LB2D::reconstructOriginalGrid,
+// coreLB.hpp:1413-1552, whose interpolation is commented out); solids 0.
+void orc_pixel_domain(void* h, double* ux, double* uy, double* rho) {
+    Sim* s = static_cast<Sim*>(h);
+    const int N = s->NX;
+    std::fill(ux, ux + (size_t)N * N, 0.0);
+    std::fill(uy, uy + (size_t)N * N, 0.0);
+    std::fill(rho, rho + (size_t)N * N, 0.0);
+    for (int64_t c = 0; c < s->G.size; c++) {
+        if (s->G.gridIsBuffer[c]) continue;
+        const int w = 1 << (s->G.gridLevel[c] - 1);
+        for (int dy = 0; dy < w; dy++)
+            for (int dx = 0; dx < w; dx++) {
+                const size_t p = (size_t)(s->G.gridIJ[2 * c + 1] + dy) * N + s->G.gridIJ[2 * c] + dx;
+                ux[p] = s->ux[c];
+                uy[p] = s->uy[c];
+                rho[p] = s->rho[c];
+            }
+    }
+}
+int64_t orc_parent_child(void* h, int64_t* rows) {
+    Sim* s = static_cast<Sim*>(h);
+    if (rows) std::copy(s->G.parentChild.begin(), s->G.parentChild.end(), rows);
+    return (int64_t)s->G.parentChild.size() / 5;
+}
 int orc_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -60,6 +60,9 @@ def load(fast=False):
     lib.orc_diff_history.argtypes = [_P, _D]
     lib.orc_complete_domain.argtypes = [_P, _D, _D, _D]
     lib.orc_psi_negative.argtypes = [_P]
+    lib.orc_parent_child.restype = C.c_int64
+    lib.orc_parent_child.argtypes = [_P, _L]
+    lib.orc_pixel_domain.argtypes = [_P, _D, _D, _D]
     return lib
 
 
@@ -179,3 +182,18 @@ class Oracle:
         c = np.zeros(n, np.int32)
         self.lib.orc_links(self.h, w, _ptr(a, _L), _ptr(b, _L), _ptr(c, _I))
         return a, b, c
+
+
+def oracle_rows(o, which):
+    lib = o.lib
+    if which == "parent_child":
+        n = lib.orc_parent_child(o.h, None)
+        out = np.zeros((n, 5), np.int64)
+        lib.orc_parent_child(o.h, _ptr(out, _L))
+        return out
+    w, key = (0, 21) if which == "coarse" else (1, 12)
+    n = lib.orc_buffer(o.h, w, key, None)
+    out = np.zeros((n, 5), np.int64)
+    if n:
+        lib.orc_buffer(o.h, w, key, _ptr(out, _L))
+    return out
