@@ -63,6 +63,41 @@ struct mts_sim_s {
     double Fbody = 1e-8;
     std::vector<void*> allocs;
 
+    // ---- multi-block lattice: one site image per level
+    struct LevelDev {
+        int nx = 0, ny = 0, level = 1;
+        int64_t stride = 0, P = 0, ncell = 0;
+        uint32_t* d_meta = nullptr;
+        uint16_t* d_pidx = nullptr;
+        int32_t* d_sites = nullptr;
+        void* d_A = nullptr;
+        double *d_f = nullptr, *d_rho = nullptr, *d_ux = nullptr, *d_uy = nullptr, *d_usq = nullptr, *d_usq_old = nullptr;
+    };
+    bool multi = false;
+    MultiGeometry mg;
+    std::vector<LevelImage> limg;
+    std::vector<LevelDev> lv;
+    std::vector<CellParam> mparams;
+    CoarseRow* d_crows = nullptr;
+    FineRow* d_frows = nullptr;
+    int64_t ncrows = 0, nfrows = 0;
+
+    LaunchCtx lctx(int l) const {
+        LaunchCtx c = ctx();
+        const LevelDev& L = lv[l];
+        c.nx = L.nx;
+        c.ny = L.ny;
+        c.stride = L.stride;
+        c.ncell = L.ncell;
+        c.meta = L.d_meta;
+        c.pidx = L.d_pidx;
+        c.cellPos = L.d_sites;
+        c.one_param = mparams.size() == 1;
+        c.fscale = (double)(1 << l);
+        c.gpIn = c.gpOut = -1;
+        return c;
+    }
+
     LaunchCtx ctx() const {
         LaunchCtx c;
         c.stream = stream;
@@ -135,6 +170,220 @@ struct mts_sim_s {
     }
 };
 
+// ---------------------------------------------------------------- multi-block
+static int upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return MTS_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+    return MTS_OK;
+}
+
+// Interface rows of coreLB.hpp:1101-1231 with every coordinate and weight
+// input formed on the host as the reference forms them.
+static void interface_rows(const MultiGeometry& G, const std::vector<LevelImage>& L, std::vector<CoarseRow>& cr,
+                           std::vector<FineRow>& fr) {
+    auto site = [&](int64_t c) -> int64_t {
+        const int l = G.level[c] - 1;
+        return (int64_t)(G.cy[c] >> l) * L[l].nx + (G.cx[c] >> l);
+    };
+    for (size_t r = 0; r + 4 < G.coarseRows.size() + 1; r += 5) {
+        CoarseRow R{};
+        const int64_t* row = &G.coarseRows[r];
+        R.parent = site(row[0]);
+        R.tc = G.taus[row[0]];
+        double tf = 0;
+        int nch = 0;
+        for (int k = 0; k < 4; k++) {
+            R.child[k] = row[1 + k] >= 0 ? site(row[1 + k]) : -1;
+            if (row[1 + k] >= 0) {
+                tf += G.taus[row[1 + k]];
+                nch++;
+            }
+        }
+        R.nch = nch;
+        R.tf = nch ? tf / (nch == 4 ? 4 : nch) : 1.0;
+        cr.push_back(R);
+    }
+    static const int top[9] = {7, 4, 8, 3, 0, 1, 6, 2, 5};
+    static const int bottom[9] = {6, 2, 5, 3, 0, 1, 7, 4, 8};
+    for (size_t r = 0; r + 4 < G.fineRows.size() + 1; r += 5) {
+        FineRow R{};
+        const int64_t* row = &G.fineRows[r];
+        const int64_t par = row[0];
+        const int64_t* pc = &G.connect[9 * par];
+        const int py = G.cy[par], px = G.cx[par];
+        const double pshift = std::pow(2.0, 2 - 2) - 0.5, half = std::pow(2, G.level[par] - 1);
+        std::vector<double> X(9), Y(9);
+        for (int ic = 0; ic < 9; ic++) {
+            const int ny_ = pc[ic] >= 0 ? G.cy[pc[ic]] : py + 2 * EY[ic];
+            const int nx_ = pc[ic] >= 0 ? G.cx[pc[ic]] : px + 2 * EX[ic];
+            if (std::abs(py - ny_) > half && py - ny_ < 0) X[ic] = py + pshift + 1 - 2;
+            else if (std::abs(py - ny_) > half && py - ny_ > 0) X[ic] = py + pshift + 1 + 2;
+            else X[ic] = ny_ + pshift + 1;
+            if (std::abs(px - nx_) > half && px - nx_ < 0) Y[ic] = px + pshift + 1 - 2;
+            else if (std::abs(px - nx_) > half && px - nx_ > 0) Y[ic] = px + pshift + 1 + 2;
+            else Y[ic] = nx_ + pshift + 1;
+        }
+        std::sort(X.begin(), X.end());
+        X.erase(std::unique(X.begin(), X.end()), X.end());
+        std::sort(Y.begin(), Y.end());
+        Y.erase(std::unique(Y.begin(), Y.end()), Y.end());
+        const bool topHalf = py < G.nx / 2;
+        for (int k = 0; k < 3; k++) R.xg[k] = Y[std::min<size_t>(k, Y.size() - 1)];
+        for (int k = 0; k < 2; k++) R.yg[k] = topHalf ? X[std::min<size_t>(k, X.size() - 1)] : X[X.size() - k - 1];
+        const int* ord = topHalf ? top : bottom;
+        for (int k = 0; k < 6; k++) R.src[k] = site(pc[ord[k]] >= 0 ? pc[ord[k]] : par);
+        for (int ch = 0; ch < 4; ch++) {
+            const int64_t cc = row[1 + ch];
+            R.child[ch] = site(cc);
+            R.qx[ch] = G.cx[cc] + 0.0 + 1;
+            R.qy[ch] = G.cy[cc] + 0.0 + 1;
+            R.ratio[ch] = G.taus[cc] / (2 * G.taus[par]);
+        }
+        fr.push_back(R);
+    }
+}
+
+static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
+                        std::string& err) {
+    s->multi = true;
+    int rc = build_multi_grid(nx, ny, pore, kn, locpore, s->ph, s->prm.periodic_y != 0, s->mg, err);
+    if (rc) return rc;
+    rc = build_level_images(s->mg, s->limg, s->mparams, s->ph, err);
+    if (rc) return rc;
+    hipError_t e = hipSetDevice(s->prm.device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
+    const size_t esz = s->prm.precision == MTS_FP32DEV ? 4 : 8;
+    int64_t Pmax = 0;
+    s->geo.nx = nx;
+    s->geo.ny = ny;
+    s->geo.ncell = s->mg.ncell;
+    s->lv.resize(s->limg.size());
+    for (size_t l = 0; l < s->limg.size(); l++) {
+        const LevelImage& I = s->limg[l];
+        mts_sim_s::LevelDev& L = s->lv[l];
+        L.nx = I.nx;
+        L.ny = I.ny;
+        L.level = I.level;
+        L.stride = I.stride;
+        L.P = (int64_t)I.nx * I.ny;
+        L.ncell = (int64_t)I.cells.size();
+        Pmax = std::max(Pmax, L.P);
+        const size_t np = (size_t)I.stride * 9;
+        if ((rc = s->alloc(&L.d_meta, L.P)) || (rc = s->alloc(&L.d_pidx, L.P)) || (rc = s->alloc(&L.d_sites, L.ncell)) ||
+            (rc = s->alloc((char**)&L.d_A, np * esz)) || (rc = s->alloc(&L.d_f, np)) || (rc = s->alloc(&L.d_rho, L.P)) ||
+            (rc = s->alloc(&L.d_ux, L.P)) || (rc = s->alloc(&L.d_uy, L.P)) || (rc = s->alloc(&L.d_usq, L.P)) ||
+            (rc = s->alloc(&L.d_usq_old, L.P)))
+            return rc;
+        if ((rc = upload(L.d_meta, I.meta.data(), L.P * 4, s->stream)) ||
+            (rc = upload(L.d_pidx, I.pidx.data(), L.P * 2, s->stream)) ||
+            (rc = upload(L.d_sites, I.sites.data(), L.ncell * 4, s->stream)))
+            return rc;
+    }
+    s->P = Pmax;
+    s->nparts = (int)std::min<int64_t>(1024, (Pmax + 255) / 256);
+    if ((rc = s->alloc(&s->d_params, s->mparams.size())) || (rc = s->alloc(&s->d_part, (size_t)4 * s->nparts)) ||
+        (rc = s->alloc(&s->d_red, 4)) || (rc = s->alloc(&s->d_neg, 1)))
+        return rc;
+    if ((rc = upload(s->d_params, s->mparams.data(), s->mparams.size() * sizeof(CellParam), s->stream))) return rc;
+    std::vector<CoarseRow> cr;
+    std::vector<FineRow> fr;
+    interface_rows(s->mg, s->limg, cr, fr);
+    s->ncrows = (int64_t)cr.size();
+    s->nfrows = (int64_t)fr.size();
+    if ((rc = s->alloc((char**)&s->d_crows, cr.size() * sizeof(CoarseRow))) ||
+        (rc = s->alloc((char**)&s->d_frows, fr.size() * sizeof(FineRow))))
+        return rc;
+    if ((rc = upload(s->d_crows, cr.data(), cr.size() * sizeof(CoarseRow), s->stream)) ||
+        (rc = upload(s->d_frows, fr.data(), fr.size() * sizeof(FineRow), s->stream)))
+        return rc;
+    upload_icsr(s->stream);
+    upload_xynorm(s->stream);
+    for (size_t l = 0; l < s->lv.size(); l++) {
+        LaunchCtx c = s->lctx((int)l);
+        launch_init(c, s->lv[l].d_f);
+        launch_macro(c, s->lv[l].d_f, s->lv[l].d_rho, s->lv[l].d_ux, s->lv[l].d_uy, s->lv[l].d_usq);
+    }
+    return s->launched();
+}
+
+// One call = nsteps coarse steps of the transport loop on the quadtree
+// (mantiisApp.hpp:239-246 with evolutionStepOfMultiBlock, coreLB.hpp:1310-1331).
+static int multi_transport(mts_sim_s* s, int nsteps) {
+    const int Lmax = (int)s->lv.size();
+    int rc;
+    for (int step = 0; step < nsteps; step++) {
+        const bool last = step == nsteps - 1;
+        if (last)
+            for (auto& L : s->lv)
+                HIP_TRY(hipMemcpyAsync(L.d_usq_old, L.d_usq, L.P * 8, hipMemcpyDeviceToDevice, s->stream));
+        for (int l = Lmax - 1; l >= 0; l--) {
+            const LaunchCtx c = s->lctx(l);
+            auto& L = s->lv[l];
+            for (int rep = 0; rep < (1 << (Lmax - 1 - l)); rep++) {
+                TransportSpec a;
+                a.fcan = L.d_f;
+                a.Aout = L.d_A;
+                a.min_rho = L.d_rho;
+                a.min_ux = L.d_ux;
+                a.min_uy = L.d_uy;
+                a.min_usq = L.d_usq;
+                if ((rc = launch_transport(c, 0, a))) return fail(rc, "operator combination has no kernel");
+                TransportSpec b;
+                b.Ain = L.d_A;
+                b.fcan = L.d_f;
+                launch_transport(c, 2, b);
+            }
+        }
+        if (Lmax > 1) {
+            auto &L1 = s->lv[0], &L2 = s->lv[1];
+            launch_coarse_interp(s->stream, s->ncrows, s->d_crows, L2.d_f, L2.stride, L1.d_f, L1.stride, L2.d_rho, L2.d_ux,
+                                 L2.d_uy, L2.d_usq);
+            launch_fine_interp(s->stream, s->nfrows, s->d_frows, L1.d_f, L1.stride, L2.d_f, L2.stride, L1.d_rho, L1.d_ux,
+                               L1.d_uy, L1.d_usq);
+        }
+        for (int l = 0; l < Lmax; l++) {
+            auto& L = s->lv[l];
+            launch_macro(s->lctx(l), L.d_f, L.d_rho, L.d_ux, L.d_uy, L.d_usq);
+        }
+        if ((rc = s->launched())) return rc;
+    }
+    double s1 = 0, s2 = 0;
+    for (int l = 0; l < Lmax; l++) {
+        double r[4];
+        const LaunchCtx c = s->lctx(l);
+        launch_reduce(c, 0, s->lv[l].d_usq, s->lv[l].d_usq_old, nullptr, s->d_part, s->nparts, s->d_red);
+        if ((rc = s->launched())) return rc;
+        HIP_TRY(hipMemcpyAsync(r, s->d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        s1 += r[0];
+        s2 += r[1];
+    }
+    s->diff = s1 / s2;
+    return MTS_OK;
+}
+
+// Downloads one per-cell field of a multi-block lattice into compact order.
+static int multi_get(mts_sim_s* s, int field, double* out) {
+    const int q = field == MTS_F_F ? 9 : 1;
+    for (size_t l = 0; l < s->lv.size(); l++) {
+        auto& L = s->lv[l];
+        const double* src = field == MTS_F_F ? L.d_f : field == MTS_F_RHO ? L.d_rho : field == MTS_F_UX ? L.d_ux
+                          : field == MTS_F_UY ? L.d_uy : field == MTS_F_USQ ? L.d_usq : field == MTS_F_USQ_OLD ? L.d_usq_old
+                                                                                                                 : nullptr;
+        if (!src) return fail(MTS_ERR_UNSUPPORTED, "field not held by multi-block lattices");
+        std::vector<double> h((size_t)(q == 9 ? L.stride * 9 : L.P));
+        HIP_TRY(hipMemcpyAsync(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        const LevelImage& I = s->limg[l];
+        for (size_t k = 0; k < I.cells.size(); k++)
+            for (int j = 0; j < q; j++)
+                out[I.cells[k] * q + j] = h[(size_t)(q == 9 ? j * L.stride : 0) + I.sites[k]];
+    }
+    return MTS_OK;
+}
+
 extern "C" {
 
 void mts_default_params(mts_params* p) {
@@ -199,7 +448,6 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
     mts_params prm;
     if (p) prm = *p;
     else mts_default_params(&prm);
-    if (prm.multiblock) return fail(MTS_ERR_UNSUPPORTED, "multi-block grids are not provided by this build");
     if (prm.precision != MTS_FP64 && prm.precision != MTS_FP32DEV) return fail(MTS_ERR_ARG, "bad precision");
     if (prm.collision < 0 || prm.collision > 2) return fail(MTS_ERR_ARG, "Incorrect keyword for collision");
     if (prm.wall < 0 || prm.wall > 2) return fail(MTS_ERR_ARG, "Incorrect keyword for wall boundary method");
@@ -218,6 +466,19 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
         return fail(MTS_ERR_PHYSICS, "PR-EOS: no single positive real root at this (T, P)");
     }
     std::string err;
+    if (prm.multiblock) {
+        if (prm.wall == MTS_WALL_MDBB) {
+            delete s;
+            return fail(MTS_ERR_UNSUPPORTED, "MDBB walls are not provided on multi-block lattices");
+        }
+        int rc = create_multi(s, nx, ny, pore, kn, locpore, err);
+        if (rc) {
+            delete s;
+            return fail(rc, err);
+        }
+        *out = s;
+        return MTS_OK;
+    }
     int rc = build_single_grid(nx, ny, pore, kn, locpore, s->ph, prm.periodic_y != 0, s->geo, err);
     if (rc) {
         delete s;
@@ -376,6 +637,18 @@ int mts_cell_count(mts_sim s, int64_t* n) {
 
 int mts_grid_info(mts_sim s, int64_t* gridID, int32_t* gridIJ, int32_t* level, int64_t* connect, int32_t* bnd) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
+    if (s->multi) {
+        const MultiGeometry& m = s->mg;
+        for (int64_t c = 0; c < m.ncell; c++) {
+            if (gridID) gridID[c] = m.gridID[c];
+            if (gridIJ) { gridIJ[2 * c] = m.cx[c]; gridIJ[2 * c + 1] = m.cy[c]; }
+            if (level) level[c] = m.level[c];
+            if (bnd) bnd[c] = m.bnd[c];
+            if (connect)
+                for (int k = 0; k < 9; k++) connect[9 * c + k] = m.connect[9 * c + k];
+        }
+        return MTS_OK;
+    }
     const Geometry& g = s->geo;
     for (int64_t c = 0; c < g.ncell; c++) {
         const int64_t p = g.cellPos[c];
@@ -394,6 +667,48 @@ int mts_grid_info(mts_sim s, int64_t* gridID, int32_t* gridIJ, int32_t* level, i
 
 int mts_links(mts_sim s, int which, int64_t* id, int64_t* loc, int32_t* ic, int64_t* count) {
     if (!s || !count || which < 0 || which > 2) return fail(MTS_ERR_ARG, "bad argument");
+    if (s->multi) {
+        // decode the level images' link words as held on the device
+        std::vector<std::vector<uint32_t>> meta(s->lv.size());
+        std::vector<std::vector<int64_t>> cellAt(s->lv.size());
+        for (size_t l = 0; l < s->lv.size(); l++) {
+            meta[l].resize(s->lv[l].P);
+            HIP_TRY(hipMemcpyAsync(meta[l].data(), s->lv[l].d_meta, s->lv[l].P * 4, hipMemcpyDeviceToHost, s->stream));
+            cellAt[l].assign(s->lv[l].P, -1);
+            for (size_t k = 0; k < s->limg[l].cells.size(); k++) cellAt[l][s->limg[l].sites[k]] = s->limg[l].cells[k];
+        }
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        std::vector<int64_t> a, b;
+        std::vector<int32_t> c;
+        const MultiGeometry& G = s->mg;
+        for (int64_t cc = 0; cc < G.ncell; cc++) {
+            const int l = G.level[cc] - 1;
+            const int nxl = s->lv[l].nx, nyl = s->lv[l].ny;
+            const int x = G.cx[cc] >> l, y = G.cy[cc] >> l;
+            const uint32_t w = meta[l][(size_t)y * nxl + x];
+            for (int k = 0; k < 9; k++) {
+                if (k == 0) {
+                    if (which == 0) { a.push_back(cc); b.push_back(cc); c.push_back(0); }
+                    continue;
+                }
+                const int xq = ((x + EX[k]) % nxl + nxl) % nxl, yq = ((y + EY[k]) % nyl + nyl) % nyl;
+                const int64_t q = (int64_t)yq * nxl + xq;
+                const bool xseam = !G.torus && (x + EX[k] < 0 || x + EX[k] >= nxl);
+                const bool linked = cellAt[l][q] >= 0 && ((meta[l][q] >> (IN_SHIFT + k - 1)) & 1u);
+                if ((which == 0 && linked && !xseam) || (which == 2 && linked && xseam)) {
+                    a.push_back(cc);
+                    b.push_back(cellAt[l][q]);
+                    c.push_back(k);
+                }
+                if (which == 1 && ((w >> (WALL_SHIFT + k - 1)) & 1u)) { a.push_back(cc); b.push_back(-1); c.push_back(k); }
+            }
+        }
+        *count = (int64_t)a.size();
+        if (id) std::copy(a.begin(), a.end(), id);
+        if (loc) std::copy(b.begin(), b.end(), loc);
+        if (ic) std::copy(c.begin(), c.end(), ic);
+        return MTS_OK;
+    }
     Geometry g = s->geo;  // decode from the words held on the device
     HIP_TRY(hipMemcpyAsync(g.meta.data(), s->d_meta, s->P * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -428,6 +743,15 @@ int mts_set_body_force(mts_sim s, double Fbody) {
 
 int mts_initialize(mts_sim s) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
+    if (s->multi) {
+        for (size_t l = 0; l < s->lv.size(); l++) {
+            LaunchCtx c = s->lctx((int)l);
+            launch_init(c, s->lv[l].d_f);
+            launch_macro(c, s->lv[l].d_f, s->lv[l].d_rho, s->lv[l].d_ux, s->lv[l].d_uy, s->lv[l].d_usq);
+        }
+        s->diff = 1.0;
+        return s->launched();
+    }
     LaunchCtx c = s->ctx();
     launch_init(c, s->d_f);
     launch_macro(c, s->d_f, s->d_rho, s->d_ux, s->d_uy, s->d_usq);
@@ -450,6 +774,7 @@ static int transport_step_phases(mts_sim s) {
 int mts_step_transport(mts_sim s, int nsteps) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
     if (nsteps <= 0) return fail(MTS_ERR_ARG, "nsteps must be positive");
+    if (s->multi) return multi_transport(s, nsteps);
     if (s->prm.wall == MTS_WALL_MDBB) {
         if (s->prm.precision != MTS_FP64) return fail(MTS_ERR_UNSUPPORTED, "MDBB wall is provided in FP64 only");
         for (int i = 0; i < nsteps; i++) {
@@ -491,6 +816,7 @@ int mts_step_transport(mts_sim s, int nsteps) {
 
 static int sc_steps(mts_sim s, int kind, int nsteps) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
+    if (s->multi) return fail(MTS_ERR_UNSUPPORTED, "multi-block lattices provide the fused transport step only");
     if (nsteps <= 0) return fail(MTS_ERR_ARG, "nsteps must be positive");
     int rc;
     if ((rc = s->ensure_sc())) return rc;
@@ -553,6 +879,7 @@ int mts_residual(mts_sim s, double* diff) {
 
 int mts_method(mts_sim s, int method, int lvl) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
+    if (s->multi) return fail(MTS_ERR_UNSUPPORTED, "multi-block lattices provide the fused transport step only");
     int rc;
     if ((rc = s->ensure_phase()) || (rc = s->ensure_sc())) return rc;
     const size_t np = (size_t)s->geo.stride * 9 * 8;
@@ -630,10 +957,11 @@ int mts_field_size(mts_sim s, int field, int64_t* n) {
 int mts_get_field(mts_sim s, int field, double* out) {
     if (!s || !out) return fail(MTS_ERR_ARG, "null argument");
     if (field == MTS_F_TAUS || field == MTS_F_KN) {
-        const auto& v = field == MTS_F_TAUS ? s->geo.taus : s->geo.kn;
+        const auto& v = s->multi ? (field == MTS_F_TAUS ? s->mg.taus : s->mg.kn) : (field == MTS_F_TAUS ? s->geo.taus : s->geo.kn);
         std::copy(v.begin(), v.end(), out);
         return MTS_OK;
     }
+    if (s->multi) return multi_get(s, field, out);
     int rc;
     if (field == MTS_F_FTEMP || field == MTS_F_FEQ || field == MTS_F_FNEQ || field == MTS_F_FB)
         if ((rc = s->ensure_phase())) return rc;
@@ -659,6 +987,7 @@ int mts_get_field(mts_sim s, int field, double* out) {
 
 int mts_set_field(mts_sim s, int field, const double* in) {
     if (!s || !in) return fail(MTS_ERR_ARG, "null argument");
+    if (s->multi) return fail(MTS_ERR_UNSUPPORTED, "fields of multi-block lattices are read-only");
     if (field == MTS_F_TAUS || field == MTS_F_KN) return fail(MTS_ERR_ARG, "read-only field");
     int rc;
     if (field == MTS_F_FTEMP || field == MTS_F_FEQ || field == MTS_F_FNEQ || field == MTS_F_FB)
@@ -683,6 +1012,32 @@ int mts_set_field(mts_sim s, int field, const double* in) {
 
 int mts_complete_domain(mts_sim s, double* ux, double* uy, double* rho) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
+    if (s->multi) {
+        // each pixel takes the value of the non-buffer cell covering it (the
+        // assignment of coreLB.hpp:1413-1552); wall pixels 0
+        const int nx = s->mg.nx, ny = s->mg.ny;
+        const int64_t n = s->mg.ncell;
+        std::vector<double> a(n), b(n), c(n);
+        int rc;
+        if ((rc = multi_get(s, MTS_F_UX, a.data())) || (rc = multi_get(s, MTS_F_UY, b.data())) ||
+            (rc = multi_get(s, MTS_F_RHO, c.data())))
+            return rc;
+        double* outs[3] = {ux, uy, rho};
+        const std::vector<double>* vals[3] = {&a, &b, &c};
+        for (int k = 0; k < 3; k++)
+            if (outs[k]) std::fill(outs[k], outs[k] + (size_t)nx * ny, 0.0);
+        for (int64_t cc = 0; cc < n; cc++) {
+            if (s->mg.isBuffer[cc]) continue;
+            const int w = 1 << (s->mg.level[cc] - 1);
+            for (int dy = 0; dy < w; dy++)
+                for (int dx = 0; dx < w; dx++) {
+                    const size_t p = (size_t)(s->mg.cy[cc] + dy) * nx + s->mg.cx[cc] + dx;
+                    for (int k = 0; k < 3; k++)
+                        if (outs[k]) outs[k][p] = (*vals[k])[cc];
+                }
+        }
+        return MTS_OK;
+    }
     double* outs[3] = {ux, uy, rho};
     double* src[3] = {s->d_ux, s->d_uy, s->d_rho};
     for (int k = 0; k < 3; k++) {
@@ -696,6 +1051,21 @@ int mts_complete_domain(mts_sim s, double* ux, double* uy, double* rho) {
 
 int mts_diagnostics(mts_sim s, double* out4) {
     if (!s || !out4) return fail(MTS_ERR_ARG, "null argument");
+    if (s->multi) {  // sums over the cells of every level, weighted by the pixels each covers
+        double acc[4] = {0, 0, 0, 0};
+        for (size_t l = 0; l < s->lv.size(); l++) {
+            double r[4];
+            launch_reduce(s->lctx((int)l), 2, s->lv[l].d_rho, s->lv[l].d_ux, s->lv[l].d_uy, s->d_part, s->nparts, s->d_red);
+            int rc = s->launched();
+            if (rc) return rc;
+            HIP_TRY(hipMemcpyAsync(r, s->d_red, 32, hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            const double wgt = (double)(1 << (2 * l));
+            for (int k = 0; k < 4; k++) acc[k] += wgt * r[k];
+        }
+        std::copy(acc, acc + 4, out4);
+        return MTS_OK;
+    }
     return s->reduce(2, s->d_rho, s->d_ux, s->d_uy, out4);
 }
 

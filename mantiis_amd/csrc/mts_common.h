@@ -58,6 +58,7 @@ struct Physics {
     bool cubic_ok = false;
 };
 void compute_physics(Physics& ph);
+CellParam make_param(double taus, double kn, const Physics& ph);
 
 // Host-side description of one single-block lattice.
 struct Geometry {

@@ -70,6 +70,24 @@ void compute_physics(Physics& ph) {
 
 static int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// Relaxation rates and wall coefficients of one (tau_s, Kn) pair, with the
+// reference's formulas (coreLB.hpp:603-604, 757-759, 925, 961).
+CellParam make_param(double ts, double K, const Physics& ph) {
+    CellParam cp{};
+    const double A2 = 0.9;
+    const double tq = 0.5 + (3.0 + M_PI * (2.0 * ts - 1.0) * (2.0 * ts - 1.0) * A2) / (8.0 * (2.0 * ts - 1.0));
+    const double base = 1.0 / std::sqrt((double)(ph.mfp / ph.Cl * 1.0 / (1.0 + 2.0 * K))) * 2.0;
+    const double td = 0.5 + (3.0 / 2.0) * std::sqrt((double)3.0) * 1.0 / (base * base);
+    cp.taus = ts;
+    cp.kn = K;
+    cp.s_s = 1.0 / ts;
+    cp.s_q = 1.0 / tq;
+    cp.s_d = 1.0 / td;
+    cp.r_srbb = 0.4987 * std::pow(K, -0.131) - 0.25;
+    cp.r_mdbb = 0.3222 * std::pow(K, -0.217) - 0.25;
+    return cp;
+}
+
 int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
                       const Physics& ph, bool torus, Geometry& g, std::string& err) {
     if (nx < 4 || ny < 4) {
@@ -154,19 +172,7 @@ int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, con
                 return MTS_ERR_UNSUPPORTED;
             }
             uniq.emplace(key, id);
-            CellParam cp{};
-            const double ts = g.taus[c];
-            const double A2 = 0.9;
-            const double tq = 0.5 + (3.0 + M_PI * (2.0 * ts - 1.0) * (2.0 * ts - 1.0) * A2) / (8.0 * (2.0 * ts - 1.0));
-            const double base = 1.0 / std::sqrt((double)(ph.mfp / ph.Cl * 1.0 / (1.0 + 2.0 * K))) * 2.0;
-            const double td = 0.5 + (3.0 / 2.0) * std::sqrt((double)3.0) * 1.0 / (base * base);
-            cp.taus = ts;
-            cp.kn = K;
-            cp.s_s = 1.0 / ts;
-            cp.s_q = 1.0 / tq;
-            cp.s_d = 1.0 / td;
-            cp.r_srbb = 0.4987 * std::pow(K, -0.131) - 0.25;
-            cp.r_mdbb = 0.3222 * std::pow(K, -0.217) - 0.25;
+            const CellParam cp = make_param(g.taus[c], K, ph);
             g.params.push_back(cp);
         } else
             id = it->second;

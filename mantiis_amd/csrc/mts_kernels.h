@@ -20,7 +20,8 @@ struct LaunchCtx {
     int coll, wall;
     double rho0, rhoin, Fbody, Gff, Gfs, T_lu, a_alpha_lu;
     int64_t gpIn, gpOut;
-    bool one_param;  // a single (tau_s, Kn) pair: the per-site index is not read
+    bool one_param;       // a single (tau_s, Kn) pair: the per-site index is not read
+    double fscale = 1.0;  // force-shift scale 2^(L-1) of the lattice level
 };
 
 struct TransportSpec {
@@ -29,6 +30,7 @@ struct TransportSpec {
     double* fcan = nullptr;
     double *rho = nullptr, *ux = nullptr, *uy = nullptr, *usq = nullptr;
     double* usq_old = nullptr;
+    const double *min_rho = nullptr, *min_ux = nullptr, *min_uy = nullptr, *min_usq = nullptr;
 };
 
 struct ScSpec {
@@ -55,6 +57,26 @@ void launch_gather(const LaunchCtx& c, const double* dense, int q, double* compa
 void launch_scatter(const LaunchCtx& c, double* dense, int q, const double* compact);
 void launch_init(const LaunchCtx& c, double* fcan);
 void launch_macro(const LaunchCtx& c, const double* fcan, double* rho, double* ux, double* uy, double* usq);
+// Multi-block interface (coreLB.hpp:1101-1231) on the canonical FP64 arrays.
+struct CoarseRow {  // coarse buffer site (level 2) and its 4 children (level-1 sites, -1 absent)
+    int64_t parent, child[4];
+    double tc;       // tau of the coarse buffer
+    double tf;       // mean tau of the children as the reference forms it
+    int nch;
+};
+struct FineRow {  // fine buffer children of one coarse cell
+    int64_t child[4];  // level-1 sites
+    int64_t src[6];    // level-2 sites of the 3x2 interpolation nodes Z[3*b + a]
+    double xg[3], yg[2];
+    double qx[4], qy[4];
+    double ratio[4];  // tau_f / (2 tau_c)
+};
+void launch_coarse_interp(hipStream_t st, int64_t nrows, const CoarseRow* rows, double* f2, int64_t stride2,
+                          const double* f1, int64_t stride1, const double* rho2, const double* ux2, const double* uy2,
+                          const double* usq2);
+void launch_fine_interp(hipStream_t st, int64_t nrows, const FineRow* rows, double* f1, int64_t stride1, const double* f2,
+                        int64_t stride2, const double* rho1, const double* ux1, const double* uy1, const double* usq1);
+void launch_pixel_out(hipStream_t st, int64_t npix, const int64_t* src, const double* const* level_arrays, double* out);
 // Unfused LB2D methods on the canonical FP64 arrays (mts_phase.hip).
 struct PhaseArrays {
     double *f, *ftemp, *feq, *fneq, *fb;

@@ -66,7 +66,9 @@ __device__ __forceinline__ constexpr double Mi(int k, int v) {
     return MI[k * 9 + v];
 }
 
-__constant__ signed char c_icsr[MTS_NTYPES][9];
+// icsr table (utils.hpp:749-767), 0-based; row MTS_NTYPES is the typeless
+// wall link of a buffer cell (full bounce-back: target = opposite).
+__constant__ signed char c_icsr[MTS_NTYPES + 1][9];
 
 // Site helpers ---------------------------------------------------------------
 struct Lat {
@@ -119,8 +121,8 @@ struct Macro64 {
     double rho, ux, uy, usq;
 };
 
-// coreLB.hpp:1018-1054 (force shift along x scaled by 2^(L-1), L = 1 here)
-__device__ __forceinline__ Macro64 moments64(const double* f, double shiftx, double shifty, bool level_scale) {
+// coreLB.hpp:1018-1054 (force shift along x scaled by 2^(L-1): scale)
+__device__ __forceinline__ Macro64 moments64(const double* f, double shiftx, double scale) {
     double r = 0.0, jx = 0.0, jy = 0.0;
 #pragma unroll
     for (int j = 0; j < 9; j++) {
@@ -132,8 +134,8 @@ __device__ __forceinline__ Macro64 moments64(const double* f, double shiftx, dou
     }
     Macro64 m;
     m.rho = r;
-    m.ux = jx / r + shiftx / (2.0 * r) * 1.0;
-    m.uy = level_scale ? jy / r + shifty / (2.0 * r) * 1.0 : jy / r;
+    m.ux = jx / r + shiftx / (2.0 * r) * scale;
+    m.uy = jy / r;
     m.usq = m.ux * m.ux + m.uy * m.uy;
     return m;
 }
@@ -313,9 +315,12 @@ struct TransportArgs {
     const uint16_t* pidx;
     const CellParam* params;
     Lat L;
-    double rho0, Fbody;
+    double rho0, Fbody, fscale;   // fscale = 2^(L-1) of the level
     double *rho, *ux, *uy, *usq;  // moments output (mode 2) or nullptr
     double* usq_old;              // |u|^2 of the moments this pass computes (modes 0/1) or nullptr
+    // mode 0: equilibrium from these moments instead of f's own (the feq of
+    // coreLB.hpp:420-430 computed once per coarse step, stale on fine substeps)
+    const double *min_rho, *min_ux, *min_uy, *min_usq;
 };
 
 template <class S, int MODE, int COLL, int WALL>
@@ -334,16 +339,21 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs a) {
     } else {
         gather<S, WALL>((const S*)a.Ain, a.L, x, y, p, w, ep, a.rho0, f);
     }
-    Macro64 m = moments64(f, a.Fbody, 0.0, false);
     if (MODE == 2) {
 #pragma unroll
         for (int j = 0; j < 9; j++) a.fcan[(int64_t)j * a.L.stride + p] = f[j];
-        a.rho[p] = m.rho;
-        a.ux[p] = m.ux;
-        a.uy[p] = m.uy;
-        a.usq[p] = m.usq;
+        if (a.rho) {
+            const Macro64 m = moments64(f, a.Fbody, a.fscale);
+            a.rho[p] = m.rho;
+            a.ux[p] = m.ux;
+            a.uy[p] = m.uy;
+            a.usq[p] = m.usq;
+        }
         return;
     }
+    Macro64 m;
+    if (MODE == 0 && a.min_rho) m = Macro64{a.min_rho[p], a.min_ux[p], a.min_uy[p], a.min_usq[p]};
+    else m = moments64(f, a.Fbody, a.fscale);
     if (a.usq_old) a.usq_old[p] = m.usq;
     collide64<COLL>(f, m, ep);
     S* out = (S*)a.Aout;
@@ -638,14 +648,14 @@ __global__ void k_init(double* fcan, const uint32_t* meta, int64_t P, int64_t st
 }
 
 // Moments of the canonical f (calculateMacroscopicPorperties, coreLB.hpp:1018-1054).
-__global__ void k_macro(const double* fcan, const uint32_t* meta, int64_t P, int64_t stride, double Fbody, double* rho,
-                        double* ux, double* uy, double* usq) {
+__global__ void k_macro(const double* fcan, const uint32_t* meta, int64_t P, int64_t stride, double Fbody, double fscale,
+                        double* rho, double* ux, double* uy, double* usq) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P || !(meta[p] & FLUID_BIT)) return;
     double f[9];
 #pragma unroll
     for (int j = 0; j < 9; j++) f[j] = fcan[(int64_t)j * stride + p];
-    Macro64 m = moments64(f, Fbody, 0.0, false);
+    Macro64 m = moments64(f, Fbody, fscale);
     rho[p] = m.rho;
     ux[p] = m.ux;
     uy[p] = m.uy;
@@ -660,9 +670,10 @@ __global__ void k_macro(const double* fcan, const uint32_t* meta, int64_t P, int
 using namespace dev;
 
 void upload_icsr(hipStream_t st) {
-    signed char t[MTS_NTYPES][9];
+    signed char t[MTS_NTYPES + 1][9];
     for (int i = 0; i < MTS_NTYPES; i++)
         for (int j = 0; j < 9; j++) t[i][j] = MTS_ICSR[i][j] < 0 ? 0 : MTS_ICSR[i][j];
+    for (int j = 0; j < 9; j++) t[MTS_NTYPES][j] = (signed char)OPP[j];
     (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_icsr), t, sizeof(t), 0, hipMemcpyHostToDevice, st);
 }
 
@@ -700,11 +711,16 @@ int launch_transport(const LaunchCtx& c, int mode, const TransportSpec& s) {
     a.L = Lat{c.nx, c.ny, c.stride};
     a.rho0 = c.rho0;
     a.Fbody = c.Fbody;
+    a.fscale = c.fscale;
     a.rho = s.rho;
     a.ux = s.ux;
     a.uy = s.uy;
     a.usq = s.usq;
     a.usq_old = s.usq_old;
+    a.min_rho = s.min_rho;
+    a.min_ux = s.min_ux;
+    a.min_uy = s.min_uy;
+    a.min_usq = s.min_usq;
     if (c.fp32) return launch_transport32(c, mode, s);
     if (mode == 0) return dispatch_t<double, 0>(c, a, c.coll, c.wall);
     if (mode == 1) return dispatch_t<double, 1>(c, a, c.coll, c.wall);
@@ -786,10 +802,11 @@ void launch_init(const LaunchCtx& c, double* fcan) {
 }
 void launch_macro(const LaunchCtx& c, const double* fcan, double* rho, double* ux, double* uy, double* usq) {
     const int64_t P = (int64_t)c.nx * c.ny;
-    hipLaunchKernelGGL(k_macro, dim3((P + 255) / 256), dim3(256), 0, c.stream, fcan, c.meta, P, c.stride, c.Fbody, rho,
-                       ux, uy, usq);
+    hipLaunchKernelGGL(k_macro, dim3((P + 255) / 256), dim3(256), 0, c.stream, fcan, c.meta, P, c.stride, c.Fbody,
+                       c.fscale, rho, ux, uy, usq);
 }
 
 }  // namespace mts
 #include "mts_phase.inc"
 #include "mts_fast.inc"
+#include "mts_multi.inc"

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <unordered_map>
 
 #include "mantiis_hip.h"
@@ -365,6 +366,88 @@ int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, cons
         const double K = kn[p], Lp = locpore[p] / std::pow(2.0, G.level[c] - 1);
         G.taus[c] = 0.5 + std::sqrt(6.0 / M_PI) * Lp * K / (1 + 2 * K);
         G.kn[c] = K;
+    }
+    return MTS_OK;
+}
+
+}  // namespace mts
+
+namespace mts {
+
+int build_level_images(const MultiGeometry& G, std::vector<LevelImage>& Ls, std::vector<CellParam>& params,
+                       const Physics& ph, std::string& err) {
+    Ls.assign(G.maxLevel, LevelImage());
+    for (int l = 0; l < G.maxLevel; l++) {
+        LevelImage& L = Ls[l];
+        L.level = l + 1;
+        L.nx = G.nx >> l;
+        L.ny = G.ny >> l;
+        const int64_t P = (int64_t)L.nx * L.ny;
+        L.stride = (P + 255) / 256 * 256;
+        L.meta.assign(P, 0);
+        L.pidx.assign(P, 0);
+    }
+    auto site = [&](int64_t c) -> int64_t {
+        const int l = G.level[c] - 1;
+        return (int64_t)(G.cy[c] >> l) * Ls[l].nx + (G.cx[c] >> l);
+    };
+    // parameter table over distinct (tau_s, Kn)
+    std::map<std::pair<double, double>, int> uniq;
+    params.clear();
+    for (int64_t c = 0; c < G.ncell; c++) {
+        LevelImage& L = Ls[G.level[c] - 1];
+        const int64_t s = site(c);
+        L.cells.push_back(c);
+        L.sites.push_back((int32_t)s);
+        auto key = std::make_pair(G.taus[c], G.kn[c]);
+        auto it = uniq.find(key);
+        int id;
+        if (it == uniq.end()) {
+            id = (int)params.size();
+            if (id > 65535) {
+                err = "more than 65536 distinct (tau_s, Kn) pairs";
+                return MTS_ERR_UNSUPPORTED;
+            }
+            uniq.emplace(key, id);
+            params.push_back(make_param(G.taus[c], G.kn[c], ph));
+        } else
+            id = it->second;
+        L.pidx[s] = (uint16_t)id;
+    }
+    const int half = 1 << (G.maxLevel - 1);
+    for (int64_t c = 0; c < G.ncell; c++) {
+        LevelImage& L = Ls[G.level[c] - 1];
+        const int64_t s = site(c);
+        const int bt = G.bnd[c];
+        uint32_t w = FLUID_BIT;
+        uint32_t wall = 0;
+        for (int ic = 1; ic < 9; ic++) {
+            const int64_t nb = G.connect[c * 9 + ic];
+            if (nb >= 0) {
+                bool xp = std::abs(G.cx[c] - G.cx[nb]) > half, yp = std::abs(G.cy[c] - G.cy[nb]) > half;
+                if (G.torus) xp = yp = false;
+                if (!xp && !yp) {
+                    if (bt == -1 || MTS_IFSTREAM[bt][ic]) {
+                        Ls[G.level[nb] - 1].meta[site(nb)] |= 1u << (IN_SHIFT + ic - 1);
+                        w |= 1u << (OUT_SHIFT + ic - 1);
+                    } else
+                        wall |= 1u << (ic - 1);
+                } else if (xp) {
+                    Ls[G.level[nb] - 1].meta[site(nb)] |= 1u << (IN_SHIFT + ic - 1);
+                    w |= 1u << (OUT_SHIFT + ic - 1);
+                }
+            } else if (nb == -2) {
+                if (!(bt == -1 || MTS_IFSTREAM[bt][ic])) wall |= 1u << (ic - 1);
+            } else {  // wall pixel; a typeless (buffer) cell makes it a wall link
+                if (bt < 0 || !MTS_IFSTREAM[bt][ic]) wall |= 1u << (ic - 1);
+            }
+        }
+        const uint32_t type = bt >= 0 ? (uint32_t)bt : (wall ? (uint32_t)MTS_NTYPES : TYPE_NONE);
+        w |= wall << WALL_SHIFT;
+        w |= type << TYPE_SHIFT;
+        const int xl = G.cx[c] >> (G.level[c] - 1);
+        if (!G.torus && (xl == 0 || xl == L.nx - 1)) w |= SEAM_BIT;
+        L.meta[s] |= w;
     }
     return MTS_OK;
 }

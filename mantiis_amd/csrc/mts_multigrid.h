@@ -27,3 +27,23 @@ int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, cons
                      bool torus, MultiGeometry& G, std::string& err);
 
 }  // namespace mts
+
+namespace mts {
+
+// One level of a multi-block lattice as a dense site image.
+struct LevelImage {
+    int level = 1;           // 1 finest
+    int nx = 0, ny = 0;      // image size (pixels >> (level-1))
+    int64_t stride = 0;
+    std::vector<uint32_t> meta;   // link words (mts_common.h)
+    std::vector<uint16_t> pidx;   // parameter index per site
+    std::vector<int64_t> cells;   // global cells of this level, in global order
+    std::vector<int32_t> sites;   // their sites
+};
+
+// Link words and parameter indices of every level image (the classification
+// of coreLB.hpp:292-356 applied to the multi-block connectivity).
+int build_level_images(const MultiGeometry& G, std::vector<LevelImage>& L, std::vector<CellParam>& params,
+                       const Physics& ph, std::string& err);
+
+}  // namespace mts

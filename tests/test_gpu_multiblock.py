@@ -1,0 +1,75 @@
+"""Quadtree multi-block transport (coarse step: level-2 collide/stream,
+two level-1 substeps with the step's stored equilibrium, interface
+interpolation, moments) of the HIP library against the oracle."""
+import numpy as np
+import pytest
+
+import _oracle as O
+from _gpu_util import banner, rel_l2
+from mantiis_amd import geometry as G
+from mantiis_amd.hip import Lattice
+
+pytestmark = pytest.mark.gpu
+MFP_DX = 0.7730846760658538
+
+
+def box64():
+    N = 64
+    p = np.ones((N, N), np.int32)
+    p[0, :] = p[-1, :] = p[:, 0] = p[:, -1] = 0
+    yy, xx = np.mgrid[0:N, 0:N]
+    p[(xx - 32) ** 2 + (yy - 30) ** 2 <= 36] = 0
+    return G.sanitize(p)
+
+
+CASES = [("box64_reference", box64, False), ("disks128_torus", lambda: G.disk_packing(128, 0.7, 3, 8, seed=3), True)]
+
+
+@pytest.mark.parametrize("name,geom,torus", CASES, ids=[c[0] for c in CASES])
+def test_multiblock_links_and_fp64_steps(name, geom, torus):
+    banner(f"multi-block {name} FP64 links + 60 steps")
+    pore = geom()
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    o = O.Oracle(pore, kn, lp, multiblock=True, periodic_y=torus, Fbody=1e-6)
+    g = Lattice(pore, kn, lp, precision="fp64", multiblock=1, periodic_y=int(torus), Fbody=1e-6)
+    assert g.ncell == o.ncell
+    for which in ("stream", "boundary", "periodic"):
+        a, b = o.links(which), g.links(which)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2]), which
+        if which != "boundary":
+            assert np.array_equal(a[1], b[1]), which
+    o.run("transport", 60, 0.0)
+    g.step("transport", 60)
+    d = np.abs(o.get("f") - g.get("f")).max()
+    print("max |f| diff", d, "residual", g.residual(), o.diff_history()[-1])
+    assert d <= 1e-15
+    assert abs(g.residual() - o.diff_history()[-1]) <= 1e-12 * o.diff_history()[-1]
+    ux, uy, rho = g.complete_domain()
+    oux, ouy, orho = (np.zeros(pore.size) for _ in range(3))
+    o.lib.orc_pixel_domain(o.h, oux.ctypes.data_as(O._D), ouy.ctypes.data_as(O._D), orho.ctypes.data_as(O._D))
+    assert np.array_equal(ux, oux) and np.array_equal(rho, orho)
+
+
+def test_multiblock_fp32_against_fp64_oracle():
+    banner("multi-block disks128 torus FP32 vs FP64 oracle 200 steps")
+    pore = G.disk_packing(128, 0.7, 3, 8, seed=3)
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    o = O.Oracle(pore, kn, lp, multiblock=True, periodic_y=True, Fbody=1e-6)
+    g = Lattice(pore, kn, lp, precision="fp32dev", multiblock=1, periodic_y=1, Fbody=1e-6)
+    o.run("transport", 200, 0.0)
+    g.step("transport", 200)
+    e = {k: rel_l2(g.get(k), o.get(k)) for k in ("rho", "ux", "uy")}
+    print("FP32 multi-block rel-l2", e)
+    assert e["rho"] < 1e-7 and e["ux"] < 2e-3 and e["uy"] < 5e-3
+
+
+def test_multiblock_large_torus_runs():
+    banner("multi-block disks512 torus FP32 1000 steps")
+    pore = G.disk_packing(512, 0.6, 4, 12, seed=7)
+    kn, lp = G.uniform_kn(pore, 0.5, MFP_DX)
+    g = Lattice(pore, kn, lp, precision="fp32dev", multiblock=1, periodic_y=1, Fbody=1e-6)
+    d0 = g.diagnostics()
+    g.step("transport", 1000)
+    d1 = g.diagnostics()
+    print("mass drift", (d1[0] - d0[0]) / d0[0], "flux", d1[1], "residual", g.residual())
+    assert np.isfinite(d1).all() and abs(d1[0] - d0[0]) / d0[0] < 1e-4 and d1[1] > 0
