@@ -49,6 +49,11 @@ struct mts_sim_s {
     int32_t* d_cellPos = nullptr;
     void* d_A = nullptr;
     void* d_B = nullptr;
+    // FP32 carried populations live in cell order (single block)
+    CompactImage cimg_h;
+    std::vector<float> cpar_h;
+    CompactDev cimg;
+    bool use_cimg = false;
     double* d_f = nullptr;
     double *d_rho = nullptr, *d_rho_old = nullptr, *d_ux = nullptr, *d_uy = nullptr, *d_usq = nullptr,
            *d_usq_old = nullptr;
@@ -95,6 +100,7 @@ struct mts_sim_s {
         c.one_param = mparams.size() == 1;
         c.fscale = (double)(1 << l);
         c.gpIn = c.gpOut = -1;
+        c.cimg = nullptr;
         return c;
     }
 
@@ -122,6 +128,7 @@ struct mts_sim_s {
         c.gpIn = geo.gpIn;
         c.gpOut = geo.gpOut;
         c.one_param = geo.params.size() == 1;
+        c.cimg = use_cimg ? &cimg : nullptr;
         return c;
     }
     template <class T>
@@ -175,6 +182,58 @@ static int upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
     if (bytes == 0) return MTS_OK;
     hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+    return MTS_OK;
+}
+
+// Device copy of the cell-order image of a single-block lattice.
+static int upload_compact(mts_sim_s* s) {
+    const CompactImage& h = s->cimg_h;
+    CompactDev& d = s->cimg;
+    uint32_t* cw;
+    int32_t *rr, *aux, *cidx, *segs;
+    uint16_t* pidx;
+    uint8_t* outm;
+    int rc;
+    if ((rc = s->alloc(&cw, h.ncell)) || (rc = s->alloc(&rr, 2 * h.ncell)) ||
+        (rc = s->alloc(&pidx, h.ncell)) || (rc = s->alloc(&outm, h.ncell)) ||
+        (rc = s->alloc(&aux, std::max<size_t>(h.aux.size(), 1))) || (rc = s->alloc(&cidx, h.cidx.size())) ||
+        (rc = s->alloc(&segs, h.segs.size())))
+        return rc;
+    if ((rc = upload(cw, h.cw.data(), h.ncell * 4, s->stream)) || (rc = upload(rr, h.rr.data(), h.ncell * 8, s->stream)) ||
+        (rc = upload(pidx, h.pidx.data(), h.ncell * 2, s->stream)) ||
+        (rc = upload(outm, h.outm.data(), h.ncell, s->stream)) ||
+        (rc = upload(aux, h.aux.data(), h.aux.size() * 4, s->stream)) ||
+        (rc = upload(cidx, h.cidx.data(), h.cidx.size() * 4, s->stream)) ||
+        (rc = upload(segs, h.segs.data(), h.segs.size() * 4, s->stream)))
+        return rc;
+    upload_spec(h.spec, s->stream);
+    // FP32 parameter records: s_s, s_q, s_d, r (hi + lo)
+    s->cpar_h.assign(s->geo.params.size() * 8, 0.0f);
+    for (size_t i = 0; i < s->geo.params.size(); i++) {
+        const CellParam& q = s->geo.params[i];
+        float* o = &s->cpar_h[i * 8];
+        o[0] = (float)q.s_s;
+        o[1] = (float)q.s_q;
+        o[2] = (float)q.s_d;
+        o[3] = (float)q.r_srbb;
+        o[4] = (float)(q.r_srbb - (double)o[3]);
+    }
+    float* cpar;
+    if ((rc = s->alloc(&cpar, s->cpar_h.size())) ||
+        (rc = upload(cpar, s->cpar_h.data(), s->cpar_h.size() * 4, s->stream)))
+        return rc;
+    d.cpar = cpar;
+    d.cw = cw;
+    d.rr = rr;
+    d.pidx = pidx;
+    d.outm = outm;
+    d.aux = aux;
+    d.cidx = cidx;
+    d.segs = segs;
+    d.ncell = h.ncell;
+    d.cstride = h.cstride;
+    d.ntx = h.ntx;
+    s->use_cimg = true;
     return MTS_OK;
 }
 
@@ -495,11 +554,13 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
     const Geometry& g = s->geo;
     s->P = (int64_t)g.nx * g.ny;
     const size_t np = (size_t)g.stride * 9;
-    const size_t esz = prm.precision == MTS_FP32DEV ? 4 : 8;
+    const bool fp32 = prm.precision == MTS_FP32DEV;
+    if (fp32) build_compact(g, s->cimg_h);
+    const size_t nA = fp32 ? (size_t)s->cimg_h.cstride * 9 * 4 : np * 8;
     s->nparts = (int)std::min<int64_t>(1024, (s->P + 255) / 256);
     if ((rc = s->alloc(&s->d_meta, s->P)) || (rc = s->alloc(&s->d_pidx, s->P)) ||
         (rc = s->alloc(&s->d_params, g.params.size())) || (rc = s->alloc(&s->d_cellPos, g.ncell)) ||
-        (rc = s->alloc((char**)&s->d_A, np * esz)) || (rc = s->alloc((char**)&s->d_B, np * esz)) ||
+        (rc = s->alloc((char**)&s->d_A, nA)) || (rc = s->alloc((char**)&s->d_B, nA)) ||
         (rc = s->alloc(&s->d_f, np)) || (rc = s->alloc(&s->d_rho, s->P)) || (rc = s->alloc(&s->d_rho_old, s->P)) ||
         (rc = s->alloc(&s->d_ux, s->P)) || (rc = s->alloc(&s->d_uy, s->P)) || (rc = s->alloc(&s->d_usq, s->P)) ||
         (rc = s->alloc(&s->d_usq_old, s->P)) || (rc = s->alloc(&s->d_part, (size_t)4 * s->nparts)) ||
@@ -513,6 +574,7 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
     if (e == hipSuccess)
         e = hipMemcpyAsync(s->d_cellPos, g.cellPos.data(), g.ncell * 4, hipMemcpyHostToDevice, s->stream);
     if (e != hipSuccess) return bail(fail(MTS_ERR_HIP, std::string("upload: ") + hipGetErrorString(e)));
+    if (fp32 && (rc = upload_compact(s))) return bail(rc);
     upload_icsr(s->stream);
     upload_xynorm(s->stream);
     *out = s;

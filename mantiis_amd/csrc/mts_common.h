@@ -81,6 +81,43 @@ struct Geometry {
 int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
                       const Physics& ph, bool torus, Geometry& g, std::string& err);
 
+// ---------------------------------------------------------------- cell order
+// The FP32 fused passes carry the populations in cell order (the fluid sites
+// in row-major order) and run one thread per cell.  Per cell:
+//   cw (u32): bits 0..7 IN, 8..15 WALL (as in the link word), 16..21 TYPE,
+//             bit 22 COMPLEX, bit 23 (x, y-1) is a cell, bit 24 (x, y+1) is a
+//             cell, bits 25..30 x mod 64;
+//   ru, rd:   (interleaved in rr) the number of cells before the site (x, y+1) /
+//             (x, y-1) (y wraps),
+//             so that the sources of slots 4, 8, 7 are ru, ru-1, ru+[(x,y+1)
+//             is a cell] and those of slots 2, 5, 6 are rd, rd-1, rd+[...];
+//             the x neighbours of a cell are the cells c-1 and c+1;
+//   outm:     the OUT bits (Shan-Chen neighbour sum).
+// A COMPLEX cell (lattice columns 0 and nx-1, x-seam sites, the generalized-
+// periodic cells and cells whose wall links share a specular slot) keeps its
+// sources, position and full link word in an aux record; its ru is the record.
+constexpr uint32_t CW_WALL_SHIFT = 8, CW_TYPE_SHIFT = 16;
+constexpr uint32_t CW_COMPLEX = 1u << 22, CW_ISC_D = 1u << 23, CW_ISC_U = 1u << 24;
+constexpr uint32_t CW_X_SHIFT = 25;
+constexpr int CTILE_W = 64, CTILE_H = 16;  // Shan-Chen tile (sites)
+constexpr int AUXW = 12;                    // aux record: src[8], x, y, link word, spare
+
+struct CompactImage {
+    int64_t ncell = 0, cstride = 0, naux = 0;
+    int ntx = 0;                   // tile columns of CTILE_W sites
+    std::vector<uint32_t> cw;      // [ncell]
+    std::vector<int32_t> rr;       // [ncell][2]: ru, rd (COMPLEX: aux record, 0)
+    std::vector<uint16_t> pidx;    // [ncell]
+    std::vector<uint8_t> outm;     // [ncell]
+    std::vector<int32_t> aux;      // [naux * AUXW]
+    std::vector<int32_t> cidx;     // [nx*ny] site -> cell or -1
+    std::vector<int32_t> segs;     // [ny * (ntx+1)]: cells before the site (tx*CTILE_W, y)
+    uint32_t spec[64] = {};        // per TYPE: slot j's specular source ic in bits 4(j-1)..4j-1
+};
+
+// Builds the cell-order image of a single-block lattice.
+void build_compact(const Geometry& g, CompactImage& ci);
+
 // Decodes the link lists of coreLB.hpp:272-366 from the link words.
 void decode_links(const Geometry& g, int which, std::vector<int64_t>& id, std::vector<int64_t>& loc,
                   std::vector<int32_t>& ic);

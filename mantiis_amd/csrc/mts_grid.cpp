@@ -257,4 +257,106 @@ void decode_links(const Geometry& g, int which, std::vector<int64_t>& id, std::v
     }
 }
 
+void build_compact(const Geometry& g, CompactImage& ci) {
+    const int nx = g.nx, ny = g.ny;
+    const int64_t P = (int64_t)nx * ny, n = g.ncell;
+    ci.ncell = n;
+    ci.cstride = (n + 255) / 256 * 256;
+    ci.cidx.assign(P, -1);
+    for (int64_t c = 0; c < n; c++) ci.cidx[g.cellPos[c]] = (int32_t)c;
+    // rank[p] = cells before site p (row-major)
+    std::vector<int32_t> rank(P + 1);
+    int32_t acc = 0;
+    for (int64_t p = 0; p < P; p++) {
+        rank[p] = acc;
+        acc += ci.cidx[p] >= 0;
+    }
+    rank[P] = acc;
+    ci.ntx = (nx + CTILE_W - 1) / CTILE_W;
+    ci.segs.assign((size_t)ny * (ci.ntx + 1), 0);
+    for (int y = 0; y < ny; y++)
+        for (int tx = 0; tx <= ci.ntx; tx++) {
+            const int x = std::min(tx * CTILE_W, nx);
+            ci.segs[(size_t)y * (ci.ntx + 1) + tx] = rank[(int64_t)y * nx + x];
+        }
+    ci.cw.assign(n, 0);
+    ci.rr.assign(2 * n, 0);
+    ci.pidx.assign(n, 0);
+    ci.outm.assign(n, 0);
+    ci.aux.clear();
+    bool specSet[64] = {};
+    for (int t = 0; t < 64; t++) ci.spec[t] = 0;
+    for (int64_t c = 0; c < n; c++) {
+        const int64_t p = g.cellPos[c];
+        const int x = (int)(p % nx), y = (int)(p / nx);
+        const uint32_t w = g.meta[p];
+        const uint32_t in = (w >> IN_SHIFT) & 0xffu, out = (w >> OUT_SHIFT) & 0xffu, wall = (w >> WALL_SHIFT) & 0xffu;
+        const uint32_t t = (w >> TYPE_SHIFT) & 63u;
+        const int yu = wrap(y + 1, ny), yd = wrap(y - 1, ny);
+        const int64_t pu = (int64_t)yu * nx + x, pd = (int64_t)yd * nx + x;
+        uint32_t cw = in | (wall << CW_WALL_SHIFT) | (t << CW_TYPE_SHIFT) | ((uint32_t)(x & 63) << CW_X_SHIFT);
+        if (ci.cidx[pd] >= 0) cw |= CW_ISC_D;
+        if (ci.cidx[pu] >= 0) cw |= CW_ISC_U;
+        // specular source of each slot (SRBB, coreLB.hpp:918-933)
+        uint32_t spec = 0;
+        bool ambiguous = false;
+        for (int ic = 1; ic < 9; ic++) {
+            if (!((wall >> (ic - 1)) & 1u)) continue;
+            const int sr = t < (uint32_t)MTS_NTYPES ? MTS_ICSR[t][ic] : OPP[ic];
+            if (sr <= 0) continue;
+            if ((spec >> (4 * (sr - 1))) & 15u) ambiguous = true;
+            spec |= (uint32_t)ic << (4 * (sr - 1));
+        }
+        // rows 0 and ny-1 of a non-torus lattice lose their y-seam links
+        bool complex = (w & SEAM_BIT) || x == 0 || x == nx - 1 || (!g.torus && (y == 0 || y == ny - 1)) ||
+                       ambiguous || p == g.gpIn || p == g.gpOut || ny < 3;
+        if (!complex && wall) {
+            if (!specSet[t]) {
+                specSet[t] = true;
+                ci.spec[t] = spec;
+            } else if (ci.spec[t] != spec)
+                complex = true;
+        }
+        auto push_aux = [&]() {  // explicit record of a COMPLEX cell
+            const int32_t k = (int32_t)(ci.aux.size() / AUXW);
+            for (int j = 1; j < 9; j++) {
+                const int xs = wrap(x - EX[j], nx), ys = wrap(y - EY[j], ny);
+                const int32_t q = ci.cidx[(int64_t)ys * nx + xs];
+                ci.aux.push_back(((in >> (j - 1)) & 1u) ? q : -1);
+            }
+            ci.aux.push_back(x);
+            ci.aux.push_back(y);
+            ci.aux.push_back((int32_t)w);
+            ci.aux.push_back(0);
+            return k;
+        };
+        ci.pidx[c] = g.pidx[p];
+        ci.outm[c] = (uint8_t)out;
+        if (complex) {
+            cw |= CW_COMPLEX;
+            ci.rr[2 * c] = push_aux();
+        } else {
+            ci.rr[2 * c] = rank[pu];
+            ci.rr[2 * c + 1] = rank[pd];
+            // every streamed slot must find its source by rank arithmetic
+            const int64_t ru = rank[pu], rd = rank[pd];
+            const int64_t src[9] = {c, c - 1, rd, c + 1, ru, rd - 1, rd + ((cw & CW_ISC_D) ? 1 : 0),
+                                    ru + ((cw & CW_ISC_U) ? 1 : 0), ru - 1};
+            bool ok = true;
+            for (int j = 1; j < 9; j++) {
+                if (!((in >> (j - 1)) & 1u)) continue;
+                const int xs = wrap(x - EX[j], nx), ys = wrap(y - EY[j], ny);
+                if (ci.cidx[(int64_t)ys * nx + xs] != src[j]) ok = false;
+            }
+            if (!ok) {
+                cw |= CW_COMPLEX;
+                ci.rr[2 * c] = push_aux();
+                ci.rr[2 * c + 1] = 0;
+            }
+        }
+        ci.cw[c] = cw;
+    }
+    ci.naux = (int64_t)ci.aux.size() / AUXW;
+}
+
 }  // namespace mts

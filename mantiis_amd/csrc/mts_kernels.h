@@ -8,6 +8,20 @@
 
 namespace mts {
 
+// Device copy of a CompactImage (cell-order FP32 passes).
+struct CompactDev {
+    const uint32_t* cw = nullptr;
+    const int32_t* rr = nullptr;  // interleaved (ru, rd)
+    const uint16_t* pidx = nullptr;
+    const uint8_t* outm = nullptr;
+    const int32_t* aux = nullptr;
+    const int32_t* cidx = nullptr;
+    const int32_t* segs = nullptr;
+    const void* cpar = nullptr;  // FP32 parameter records (32 B per (tau_s, Kn) pair)
+    int64_t ncell = 0, cstride = 0;
+    int ntx = 0;
+};
+
 struct LaunchCtx {
     hipStream_t stream;
     int nx, ny;
@@ -22,6 +36,7 @@ struct LaunchCtx {
     int64_t gpIn, gpOut;
     bool one_param;       // a single (tau_s, Kn) pair: the per-site index is not read
     double fscale = 1.0;  // force-shift scale 2^(L-1) of the lattice level
+    const CompactDev* cimg = nullptr;  // FP32 passes in cell order when set
 };
 
 struct TransportSpec {
@@ -46,6 +61,10 @@ void upload_icsr(hipStream_t st);
 // FP32 production kernels (mts_fast.inc)
 int launch_transport32(const LaunchCtx& c, int mode, const TransportSpec& s);
 int launch_shanchen32(const LaunchCtx& c, int kind, int mode, const ScSpec& s);
+// FP32 production kernels in cell order (mts_compact.inc)
+void upload_spec(const uint32_t* spec64, hipStream_t st);
+int launch_ctransport(const LaunchCtx& c, int mode, const TransportSpec& s);
+int launch_cshanchen(const LaunchCtx& c, int kind, int mode, const ScSpec& s);
 // mode 0 collide-only (fcan -> Aout), 1 fused (Ain -> Aout), 2 stream-only (Ain -> fcan)
 int launch_transport(const LaunchCtx& c, int mode, const TransportSpec& s);
 // kind 0 adsorption, 1 coupled; mode as above

@@ -606,19 +606,33 @@ __global__ void __launch_bounds__(256) k_reduce(int kind, const uint32_t* meta, 
     }
 }
 
-__global__ void k_reduce_final(int kind, int nparts, const double* part, double* out) {
-    if (threadIdx.x != 0) return;
+// 256 threads: strided partial sums, then a fixed-order tree in LDS
+// (deterministic for a given partial count).
+__global__ void __launch_bounds__(256) k_reduce_final(int kind, int nparts, const double* part, double* out) {
+    __shared__ double s[4][256];
+    const int t = threadIdx.x;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-    for (int i = 0; i < nparts; i++) {
+    for (int i = t; i < nparts; i += 256) {
         r0 = kind == 1 ? fmax(r0, part[4 * i]) : r0 + part[4 * i];
         r1 += part[4 * i + 1];
         r2 += part[4 * i + 2];
         r3 += part[4 * i + 3];
     }
-    out[0] = r0;
-    out[1] = r1;
-    out[2] = r2;
-    out[3] = r3;
+    s[0][t] = r0;
+    s[1][t] = r1;
+    s[2][t] = r2;
+    s[3][t] = r3;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            s[0][t] = kind == 1 ? fmax(s[0][t], s[0][t + h]) : s[0][t] + s[0][t + h];
+            s[1][t] += s[1][t + h];
+            s[2][t] += s[2][t + h];
+            s[3][t] += s[3][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < 4) out[t] = s[t][0];
 }
 
 // Gather/scatter between the dense site image and compact cell order.
@@ -721,6 +735,7 @@ int launch_transport(const LaunchCtx& c, int mode, const TransportSpec& s) {
     a.min_ux = s.min_ux;
     a.min_uy = s.min_uy;
     a.min_usq = s.min_usq;
+    if (c.fp32 && c.cimg) return launch_ctransport(c, mode, s);
     if (c.fp32) return launch_transport32(c, mode, s);
     if (mode == 0) return dispatch_t<double, 0>(c, a, c.coll, c.wall);
     if (mode == 1) return dispatch_t<double, 1>(c, a, c.coll, c.wall);
@@ -776,6 +791,7 @@ int launch_shanchen(const LaunchCtx& c, int kind, int mode, const ScSpec& s) {
     a.Fx = s.Fx;
     a.Fy = s.Fy;
     a.psi_negative = s.psi_negative;
+    if (c.fp32 && c.cimg) return launch_cshanchen(c, kind, mode, s);
     if (c.fp32) return launch_shanchen32(c, kind, mode, s);
     sc_dispatch<double>(c, kind, mode, a);
     return 0;
@@ -785,7 +801,7 @@ void launch_reduce(const LaunchCtx& c, int kind, const double* a0, const double*
                    int nparts, double* out) {
     const int64_t P = (int64_t)c.nx * c.ny;
     hipLaunchKernelGGL(k_reduce, dim3(nparts), dim3(256), 0, c.stream, kind, c.meta, P, a0, a1, a2, part);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c.stream, kind, nparts, part, out);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, kind, nparts, part, out);
 }
 
 void launch_gather(const LaunchCtx& c, const double* dense, int q, double* compact) {
@@ -809,4 +825,5 @@ void launch_macro(const LaunchCtx& c, const double* fcan, double* rho, double* u
 }  // namespace mts
 #include "mts_phase.inc"
 #include "mts_fast.inc"
+#include "mts_compact.inc"
 #include "mts_multi.inc"
