@@ -54,6 +54,33 @@ def transport_inputs(n=1024, seed=2026):
     return pore, kn, lp
 
 
+def c2_inputs(n=1024, seed=2025):
+    """Quadtree configuration: seeded disk packing, porosity U[0.4, 0.8], Kn = 0.5."""
+    from mantiis_amd import geometry as G
+    rng = np.random.default_rng(seed)
+    phi = float(rng.uniform(0.4, 0.8))
+    pore = G.disk_packing(n, phi, 4, 12, seed=seed)
+    kn, lp = G.uniform_kn(pore, 0.5, 0.7730846760658538)
+    return pore, kn, lp, phi
+
+
+def run_to_steady(lat, tol, chunk, max_steps, max_s):
+    """Steps until the residual (relative change of the velocity magnitude
+    over one step) falls below tol; returns (steps, seconds, residual)."""
+    lat.synchronize()
+    t0 = time.perf_counter()
+    n = 0
+    r = 1.0
+    while n < max_steps and time.perf_counter() - t0 < max_s:
+        lat.step("transport", chunk)
+        n += chunk
+        r = lat.residual()
+        if r < tol:
+            break
+    lat.synchronize()
+    return n, time.perf_counter() - t0, r
+
+
 def w9():
     return np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
 
@@ -113,7 +140,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c3", choices=["c3", "transport", "c1"])
+    ap.add_argument("--workload", default="c3", choices=["c2", "c3", "transport", "c1"])
+    ap.add_argument("--steady", action="store_true", help="c2: also time the run to steady state (tol 1e-6)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--n", type=int, default=0, help="lattice side of the transport workload (default 1024)")
@@ -151,6 +179,22 @@ def main():
                     "MRT + Guo source + bounce-back, FP32 deviation storage, FP64 psi")
         kernel = "k_shanchen<float,1,MRT,BB,genperiodic>"
         dtype = "fp32 (populations) / fp64 (psi, moments)"
+    elif args.workload == "c2":
+        from mantiis_amd.hip import Lattice
+        pore, kn, lp, phi = c2_inputs(args.n or 1024)
+        lat = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
+                      multiblock=1)
+        lvl = lat.grid()["level"]
+        n1, n2 = int((lvl == 1).sum()), int((lvl == 2).sum())
+        kind, inputs = "transport", None
+        # cell updates per coarse step: two level-1 substeps and one level-2 step
+        updates = 2 * n1 + n2
+        bytes_per_cell = 84  # 72 B populations + 12 B stored (rho - rho0, ux, uy) per cell update
+        workload = (f"quadtree multi-block transport RMRT+SRBB, {lat.nx}x{lat.ny} disk packing porosity {phi:.3f} "
+                    f"(U[0.4,0.8]), Kn=0.5, torus, FP32 deviation; {n1} level-1 and {n2} level-2 cells "
+                    "(2 level-1 substeps per coarse step)")
+        kernel = "coarse step (k_cmb level substeps + interface kernels)"
+        dtype = "fp32 (populations)"
     elif args.workload == "transport":
         from mantiis_amd.hip import Lattice
         pore, kn, lp = transport_inputs(args.n or 1024)
@@ -172,10 +216,20 @@ def main():
         dtype = "fp64"
 
     n = lat.ncell
+    if args.workload != "c2":
+        updates = n
     ms, ms1 = time_steps(lat, kind, args.steps, args.warmup)
-    mlups = n * args.steps / (ms / 1e3) / 1e6
+    mlups = updates * args.steps / (ms / 1e3) / 1e6
     t_kernel_ms = (ms - ms1) / (args.steps - 1)
-    achieved = bytes_per_cell * n / (t_kernel_ms / 1e3) / 1e9
+    achieved = bytes_per_cell * updates / (t_kernel_ms / 1e3) / 1e9
+    steady = None
+    if args.workload == "c2" and args.steady:
+        from mantiis_amd.hip import Lattice
+        lat2 = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
+                       multiblock=1)
+        st, sec, r = run_to_steady(lat2, 1e-6, 200, 400000, 120.0)
+        steady = {"tol": 1e-6, "steps": st, "wall_s": round(sec, 3), "residual": r}
+        lat2.close()
     diag = lat.diagnostics()
 
     cpu = None
@@ -202,6 +256,10 @@ def main():
         "cpu_baseline": cpu,
         "mass": float(diag[0]),
     }
+    if args.workload == "c2":
+        out["config"]["cell_updates_per_step"] = updates
+    if steady is not None:
+        out["steady"] = steady
     print(json.dumps(out), flush=True)
     return 0
 

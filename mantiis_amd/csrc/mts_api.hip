@@ -85,7 +85,19 @@ struct mts_sim_s {
     std::vector<CellParam> mparams;
     CoarseRow* d_crows = nullptr;
     FineRow* d_frows = nullptr;
+    CoarseRow32* d_crows32 = nullptr;
+    FineRow32* d_frows32 = nullptr;
     int64_t ncrows = 0, nfrows = 0;
+    // FP32 multi-block state in cell order, per level: image (host and device,
+    // with and without interface overrides), carried states, stored fields
+    struct LevelC {
+        CompactImage h;
+        CompactDev plain, ovr;
+        float* A[2] = {nullptr, nullptr};
+        float* m = nullptr;
+        float* O = nullptr;
+    };
+    std::vector<LevelC> lc;
 
     LaunchCtx lctx(int l) const {
         LaunchCtx c = ctx();
@@ -185,32 +197,11 @@ static int upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
     return MTS_OK;
 }
 
-// Device copy of the cell-order image of a single-block lattice.
-static int upload_compact(mts_sim_s* s) {
-    const CompactImage& h = s->cimg_h;
-    CompactDev& d = s->cimg;
-    uint32_t* cw;
-    int32_t *rr, *aux, *cidx, *segs;
-    uint16_t* pidx;
-    uint8_t* outm;
-    int rc;
-    if ((rc = s->alloc(&cw, h.ncell)) || (rc = s->alloc(&rr, 2 * h.ncell)) ||
-        (rc = s->alloc(&pidx, h.ncell)) || (rc = s->alloc(&outm, h.ncell)) ||
-        (rc = s->alloc(&aux, std::max<size_t>(h.aux.size(), 1))) || (rc = s->alloc(&cidx, h.cidx.size())) ||
-        (rc = s->alloc(&segs, h.segs.size())))
-        return rc;
-    if ((rc = upload(cw, h.cw.data(), h.ncell * 4, s->stream)) || (rc = upload(rr, h.rr.data(), h.ncell * 8, s->stream)) ||
-        (rc = upload(pidx, h.pidx.data(), h.ncell * 2, s->stream)) ||
-        (rc = upload(outm, h.outm.data(), h.ncell, s->stream)) ||
-        (rc = upload(aux, h.aux.data(), h.aux.size() * 4, s->stream)) ||
-        (rc = upload(cidx, h.cidx.data(), h.cidx.size() * 4, s->stream)) ||
-        (rc = upload(segs, h.segs.data(), h.segs.size() * 4, s->stream)))
-        return rc;
-    upload_spec(h.spec, s->stream);
-    // FP32 parameter records: s_s, s_q, s_d, r (hi + lo)
-    s->cpar_h.assign(s->geo.params.size() * 8, 0.0f);
-    for (size_t i = 0; i < s->geo.params.size(); i++) {
-        const CellParam& q = s->geo.params[i];
+// FP32 parameter records (s_s, s_q, s_d, r as hi + lo) of a parameter table.
+static int upload_cpar(mts_sim_s* s, const std::vector<CellParam>& params, const float** out) {
+    s->cpar_h.assign(params.size() * 8, 0.0f);
+    for (size_t i = 0; i < params.size(); i++) {
+        const CellParam& q = params[i];
         float* o = &s->cpar_h[i * 8];
         o[0] = (float)q.s_s;
         o[1] = (float)q.s_q;
@@ -219,10 +210,34 @@ static int upload_compact(mts_sim_s* s) {
         o[4] = (float)(q.r_srbb - (double)o[3]);
     }
     float* cpar;
+    int rc;
     if ((rc = s->alloc(&cpar, s->cpar_h.size())) ||
         (rc = upload(cpar, s->cpar_h.data(), s->cpar_h.size() * 4, s->stream)))
         return rc;
-    d.cpar = cpar;
+    *out = cpar;
+    return MTS_OK;
+}
+
+// Device copy of a cell-order image (the host image must outlive the copies).
+static int upload_compact(mts_sim_s* s, const CompactImage& h, const float* cpar, CompactDev& d) {
+    uint32_t* cw;
+    int32_t *rr, *aux, *cidx, *segs, *cpos;
+    uint16_t* pidx;
+    uint8_t* outm;
+    int rc;
+    if ((rc = s->alloc(&cw, h.ncell)) || (rc = s->alloc(&rr, 2 * h.ncell)) ||
+        (rc = s->alloc(&pidx, h.ncell)) || (rc = s->alloc(&outm, h.ncell)) ||
+        (rc = s->alloc(&aux, std::max<size_t>(h.aux.size(), 1))) || (rc = s->alloc(&cidx, h.cidx.size())) ||
+        (rc = s->alloc(&segs, h.segs.size())) || (rc = s->alloc(&cpos, h.ncell)))
+        return rc;
+    if ((rc = upload(cw, h.cw.data(), h.ncell * 4, s->stream)) || (rc = upload(rr, h.rr.data(), h.ncell * 8, s->stream)) ||
+        (rc = upload(pidx, h.pidx.data(), h.ncell * 2, s->stream)) ||
+        (rc = upload(outm, h.outm.data(), h.ncell, s->stream)) ||
+        (rc = upload(aux, h.aux.data(), h.aux.size() * 4, s->stream)) ||
+        (rc = upload(cidx, h.cidx.data(), h.cidx.size() * 4, s->stream)) ||
+        (rc = upload(segs, h.segs.data(), h.segs.size() * 4, s->stream)) ||
+        (rc = upload(cpos, h.cellPos.data(), h.ncell * 4, s->stream)))
+        return rc;
     d.cw = cw;
     d.rr = rr;
     d.pidx = pidx;
@@ -230,10 +245,11 @@ static int upload_compact(mts_sim_s* s) {
     d.aux = aux;
     d.cidx = cidx;
     d.segs = segs;
+    d.cellPos = cpos;
+    d.cpar = cpar;
     d.ncell = h.ncell;
     d.cstride = h.cstride;
     d.ntx = h.ntx;
-    s->use_cimg = true;
     return MTS_OK;
 }
 
@@ -303,6 +319,120 @@ static void interface_rows(const MultiGeometry& G, const std::vector<LevelImage>
     }
 }
 
+// Cell-order images of the levels for the FP32 path.  Buffer cells written
+// by the interface get an override slot: coarse buffer of row r -> r, fine
+// buffer k of row r -> 4 r + k.
+static int create_multi32(mts_sim_s* s, const std::vector<CoarseRow>& cr, const std::vector<FineRow>& fr,
+                          std::string& err) {
+    const int Lmax = (int)s->limg.size();
+    // every interface site must be a cell of its level image
+    auto is_cell = [&](int l, int64_t site) {
+        const LevelImage& I = s->limg[l];
+        return site >= 0 && site < (int64_t)I.nx * I.ny && (I.meta[site] & FLUID_BIT);
+    };
+    if (Lmax > 1) {
+        for (const CoarseRow& R : cr) {
+            if (R.nch > 0 && !is_cell(1, R.parent)) return fail(MTS_ERR_GEOMETRY, "interface row: parent is not a cell");
+            for (int k = 0; k < 4; k++)
+                if (R.child[k] >= 0 && !is_cell(0, R.child[k]))
+                    return fail(MTS_ERR_GEOMETRY, "interface row: child is not a cell");
+        }
+        for (const FineRow& R : fr) {
+            for (int k = 0; k < 6; k++)
+                if (!is_cell(1, R.src[k])) return fail(MTS_ERR_GEOMETRY, "interface row: node is not a cell");
+            for (int k = 0; k < 4; k++)
+                if (!is_cell(0, R.child[k])) return fail(MTS_ERR_GEOMETRY, "interface row: buffer is not a cell");
+        }
+    }
+    s->lc.resize(Lmax);
+    const float* cpar;
+    int rc;
+    if ((rc = upload_cpar(s, s->mparams, &cpar))) return rc;
+    for (int l = 0; l < Lmax; l++) {
+        const LevelImage& I = s->limg[l];
+        auto& C = s->lc[l];
+        const int64_t P = (int64_t)I.nx * I.ny;
+        std::vector<int32_t> ovr(P, -1);
+        int64_t nslot = 0;
+        if (l == 1) {
+            for (size_t r = 0; r < cr.size(); r++)
+                if (cr[r].nch > 0) {
+                    if (ovr[cr[r].parent] >= 0) {
+                        err = "coarse buffer cell in two interface rows";
+                        return MTS_ERR_GEOMETRY;
+                    }
+                    ovr[cr[r].parent] = (int32_t)r;
+                }
+            nslot = (int64_t)cr.size();
+        } else if (l == 0 && Lmax > 1) {
+            for (size_t r = 0; r < fr.size(); r++)
+                for (int k = 0; k < 4; k++) {
+                    if (ovr[fr[r].child[k]] >= 0) {
+                        err = "fine buffer cell in two interface rows";
+                        return MTS_ERR_GEOMETRY;
+                    }
+                    ovr[fr[r].child[k]] = (int32_t)(4 * r + k);
+                }
+            nslot = 4 * (int64_t)fr.size();
+        }
+        LatticeView v{I.nx, I.ny, s->mg.torus, I.meta.data(), I.pidx.data(), -1, -1, ovr.data()};
+        build_compact(v, C.h);
+        if ((rc = upload_compact(s, C.h, cpar, C.plain))) return rc;
+        const int64_t OS = std::max<int64_t>((nslot + 63) / 64 * 64, 64);
+        if ((rc = s->alloc(&C.A[0], (size_t)C.h.cstride * 9)) || (rc = s->alloc(&C.A[1], (size_t)C.h.cstride * 9)) ||
+            (rc = s->alloc(&C.m, (size_t)C.h.cstride * 3)) || (rc = s->alloc(&C.O, (size_t)OS * 9)))
+            return rc;
+        C.plain.O = C.O;
+        C.plain.OS = OS;
+        C.plain.ovr = false;
+        C.ovr = C.plain;
+        C.ovr.ovr = true;
+    }
+    upload_spec(s->lc[0].h.spec, s->stream);
+    // interface rows in cell indices with host-formed weights
+    if (Lmax > 1) {
+        const auto& ci1 = s->lc[0].h.cidx;
+        const auto& ci2 = s->lc[1].h.cidx;
+        std::vector<CoarseRow32> c32(cr.size());
+        for (size_t r = 0; r < cr.size(); r++) {
+            const CoarseRow& R = cr[r];
+            CoarseRow32& o = c32[r];
+            o.parent = R.nch > 0 ? ci2[R.parent] : 0;
+            for (int k = 0; k < 4; k++) o.child[k] = R.child[k] >= 0 ? ci1[R.child[k]] : -1;
+            o.k = R.nch > 0 ? (float)(2 * R.tc / R.tf) : 0.0f;
+            o.nch = R.nch;
+            o.slot = (int32_t)r;
+        }
+        std::vector<FineRow32> f32(fr.size());
+        for (size_t r = 0; r < fr.size(); r++) {
+            const FineRow& R = fr[r];
+            FineRow32& o = f32[r];
+            for (int k = 0; k < 4; k++) o.child[k] = ci1[R.child[k]];
+            for (int k = 0; k < 6; k++) o.src[k] = ci2[R.src[k]];
+            for (int ch = 0; ch < 4; ch++) {
+                const double qx = R.qx[ch], qy = R.qy[ch];
+                for (int b = 0; b < 2; b++) {
+                    const double Ly = (qy - R.yg[1 - b]) / (R.yg[b] - R.yg[1 - b]);
+                    for (int a = 0; a < 3; a++) {
+                        double Lx = 1;
+                        for (int mm = 0; mm < 3; mm++)
+                            if (mm != a) Lx *= (qx - R.xg[mm]) / (R.xg[a] - R.xg[mm]);
+                        o.L[ch][3 * b + a] = (float)(Lx * Ly);
+                    }
+                }
+                o.ratio[ch] = (float)R.ratio[ch];
+            }
+        }
+        if ((rc = s->alloc((char**)&s->d_crows32, c32.size() * sizeof(CoarseRow32))) ||
+            (rc = s->alloc((char**)&s->d_frows32, f32.size() * sizeof(FineRow32))) ||
+            (rc = upload(s->d_crows32, c32.data(), c32.size() * sizeof(CoarseRow32), s->stream)) ||
+            (rc = upload(s->d_frows32, f32.data(), f32.size() * sizeof(FineRow32), s->stream)))
+            return rc;
+        HIP_TRY(hipStreamSynchronize(s->stream));  // the host rows go out of scope
+    }
+    return MTS_OK;
+}
+
 static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
                         std::string& err) {
     s->multi = true;
@@ -357,6 +487,7 @@ static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const
     if ((rc = upload(s->d_crows, cr.data(), cr.size() * sizeof(CoarseRow), s->stream)) ||
         (rc = upload(s->d_frows, fr.data(), fr.size() * sizeof(FineRow), s->stream)))
         return rc;
+    if (s->prm.precision == MTS_FP32DEV && (rc = create_multi32(s, cr, fr, err))) return rc;
     upload_icsr(s->stream);
     upload_xynorm(s->stream);
     for (size_t l = 0; l < s->lv.size(); l++) {
@@ -369,7 +500,118 @@ static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const
 
 // One call = nsteps coarse steps of the transport loop on the quadtree
 // (mantiisApp.hpp:239-246 with evolutionStepOfMultiBlock, coreLB.hpp:1310-1331).
+// FP32 quadtree step in cell order: per coarse step the coarse level does one
+// fused substep, the fine level two (the second with the stored fields of
+// the first), then the interface writes the buffer overrides.  A call
+// starts from and ends in the canonical FP64 level images.
+static int multi_transport32(mts_sim_s* s, int nsteps) {
+    const int Lmax = (int)s->lv.size();
+    int rc;
+    int cur[2] = {0, 0};
+    for (int step = 0; step < nsteps; step++) {
+        const bool first = step == 0, last = step == nsteps - 1;
+        if (Lmax == 2) {
+            // both levels' first substeps in one launch, then the second
+            // level-1 substep with the stored fields
+            MbSpec a[2];
+            LaunchCtx c[2];
+            for (int l = 0; l < 2; l++) {
+                auto& C = s->lc[l];
+                c[l] = s->lctx(l);
+                c[l].cimg = &C.ovr;
+                a[l].Ain = C.A[cur[l]];
+                a[l].Aout = C.A[cur[l] ^ 1];
+                a[l].fcan = s->lv[l].d_f;
+                a[l].m = C.m;
+                a[l].usq_old = last ? s->lv[l].d_usq_old : nullptr;
+                cur[l] ^= 1;
+            }
+            if ((rc = launch_cmultistep2(c[0], a[0], c[1], a[1], first ? 0 : 1)))
+                return fail(rc, "operator combination has no kernel");
+            auto& C = s->lc[0];
+            c[0].cimg = &C.plain;
+            MbSpec b;
+            b.Ain = C.A[cur[0]];
+            b.Aout = C.A[cur[0] ^ 1];
+            b.m = C.m;
+            launch_cmultistep(c[0], 1, 1, b);
+            cur[0] ^= 1;
+        }
+        for (int l = Lmax == 2 ? -1 : Lmax - 1; l >= 0; l--) {
+            auto& C = s->lc[l];
+            auto& L = s->lv[l];
+            LaunchCtx c = s->lctx(l);
+            c.cimg = &C.ovr;
+            MbSpec a;
+            a.Ain = C.A[cur[l]];
+            a.Aout = C.A[cur[l] ^ 1];
+            a.fcan = L.d_f;
+            a.m = C.m;
+            a.usq_old = last ? L.d_usq_old : nullptr;
+            if ((rc = launch_cmultistep(c, first ? 0 : 1, 0, a))) return fail(rc, "operator combination has no kernel");
+            cur[l] ^= 1;
+            for (int rep = 1; rep < (1 << (Lmax - 1 - l)); rep++) {
+                c.cimg = &C.plain;
+                MbSpec b;
+                b.Ain = C.A[cur[l]];
+                b.Aout = C.A[cur[l] ^ 1];
+                b.m = C.m;
+                launch_cmultistep(c, 1, 1, b);
+                cur[l] ^= 1;
+            }
+        }
+        if (Lmax > 1) {
+            IfSpec f;
+            f.A1 = s->lc[0].A[cur[0]];
+            f.A2 = s->lc[1].A[cur[1]];
+            f.m1 = s->lc[0].m;
+            f.m2 = s->lc[1].m;
+            f.O1 = s->lc[0].O;
+            f.O2 = s->lc[1].O;
+            f.crows = s->d_crows32;
+            f.frows = s->d_frows32;
+            f.ncr = s->ncrows;
+            f.nfr = s->nfrows;
+            LaunchCtx c1 = s->lctx(0), c2 = s->lctx(1);
+            c1.cimg = &s->lc[0].ovr;
+            c2.cimg = &s->lc[1].ovr;
+            if ((rc = launch_cinterface(c1, c2, f))) return fail(rc, "interface has no kernel");
+        }
+        if ((rc = s->launched())) return rc;
+    }
+    // export: post-stream state with the interface overrides -> canonical
+    for (int l = 0; l < Lmax; l++) {
+        auto& C = s->lc[l];
+        auto& L = s->lv[l];
+        LaunchCtx c = s->lctx(l);
+        c.cimg = &C.ovr;
+        TransportSpec z;
+        z.Ain = C.A[cur[l]];
+        z.fcan = L.d_f;
+        z.rho = L.d_rho;
+        z.ux = L.d_ux;
+        z.uy = L.d_uy;
+        z.usq = L.d_usq;
+        launch_ctransport(c, 2, z);
+    }
+    if ((rc = s->launched())) return rc;
+    double s1 = 0, s2 = 0;
+    for (int l = 0; l < Lmax; l++) {
+        double r[4];
+        const LaunchCtx c = s->lctx(l);
+        launch_reduce(c, 0, s->lv[l].d_usq, s->lv[l].d_usq_old, nullptr, s->d_part, s->nparts, s->d_red);
+        if ((rc = s->launched())) return rc;
+        HIP_TRY(hipMemcpyAsync(r, s->d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        s1 += r[0];
+        s2 += r[1];
+    }
+    s->diff = s1 / s2;
+    return MTS_OK;
+}
+
 static int multi_transport(mts_sim_s* s, int nsteps) {
+    if (!s->lc.empty()) return multi_transport32(s, nsteps);
     const int Lmax = (int)s->lv.size();
     int rc;
     for (int step = 0; step < nsteps; step++) {
@@ -574,7 +816,12 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
     if (e == hipSuccess)
         e = hipMemcpyAsync(s->d_cellPos, g.cellPos.data(), g.ncell * 4, hipMemcpyHostToDevice, s->stream);
     if (e != hipSuccess) return bail(fail(MTS_ERR_HIP, std::string("upload: ") + hipGetErrorString(e)));
-    if (fp32 && (rc = upload_compact(s))) return bail(rc);
+    if (fp32) {
+        const float* cpar;
+        if ((rc = upload_cpar(s, g.params, &cpar)) || (rc = upload_compact(s, s->cimg_h, cpar, s->cimg))) return bail(rc);
+        upload_spec(s->cimg_h.spec, s->stream);
+        s->use_cimg = true;
+    }
     upload_icsr(s->stream);
     upload_xynorm(s->stream);
     *out = s;

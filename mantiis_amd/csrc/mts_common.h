@@ -100,7 +100,7 @@ constexpr uint32_t CW_WALL_SHIFT = 8, CW_TYPE_SHIFT = 16;
 constexpr uint32_t CW_COMPLEX = 1u << 22, CW_ISC_D = 1u << 23, CW_ISC_U = 1u << 24;
 constexpr uint32_t CW_X_SHIFT = 25;
 constexpr int CTILE_W = 64, CTILE_H = 16;  // Shan-Chen tile (sites)
-constexpr int AUXW = 12;                    // aux record: src[8], x, y, link word, spare
+constexpr int AUXW = 12;  // aux record: src[8], x, y, link word, override slot (-1: none)
 
 struct CompactImage {
     int64_t ncell = 0, cstride = 0, naux = 0;
@@ -111,12 +111,32 @@ struct CompactImage {
     std::vector<uint8_t> outm;     // [ncell]
     std::vector<int32_t> aux;      // [naux * AUXW]
     std::vector<int32_t> cidx;     // [nx*ny] site -> cell or -1
+    std::vector<int32_t> cellPos;  // [ncell] cell -> site
     std::vector<int32_t> segs;     // [ny * (ntx+1)]: cells before the site (tx*CTILE_W, y)
     uint32_t spec[64] = {};        // per TYPE: slot j's specular source ic in bits 4(j-1)..4j-1
 };
 
-// Builds the cell-order image of a single-block lattice.
+// The specular-source table of the cell-order kernels (the same for every
+// lattice): per boundary type (52 = typeless, full bounce-back), slot j's
+// source link ic, taken when ic is a wall link of the cell.
+void canonical_spec(uint32_t spec[64]);
+
+// A site image to put in cell order: link words and parameter indices per
+// site, the generalized-periodic sites and, per site, the slot of an
+// interface override (multi-block buffer cells; nullptr or -1: none).  A cell
+// with an override slot is COMPLEX; its aux record keeps the slot.
+struct LatticeView {
+    int nx, ny;
+    bool torus;
+    const uint32_t* meta;
+    const uint16_t* pidx;
+    int64_t gpIn, gpOut;
+    const int32_t* override;
+};
+
+// Builds the cell-order image of a single-block lattice or of one level.
 void build_compact(const Geometry& g, CompactImage& ci);
+void build_compact(const LatticeView& v, CompactImage& ci);
 
 // Decodes the link lists of coreLB.hpp:272-366 from the link words.
 void decode_links(const Geometry& g, int which, std::vector<int64_t>& id, std::vector<int64_t>& loc,

@@ -257,13 +257,38 @@ void decode_links(const Geometry& g, int which, std::vector<int64_t>& id, std::v
     }
 }
 
+void canonical_spec(uint32_t spec[64]) {
+    // slot j <- the link ic that is not streamed for the type and whose
+    // specular direction is j (the first such link when two share a slot)
+    for (int t = 0; t < 64; t++) spec[t] = 0;
+    for (int t = 0; t <= MTS_NTYPES; t++)
+        for (int ic = 8; ic >= 1; ic--) {
+            const bool wl = t == MTS_NTYPES ? true : !MTS_IFSTREAM[t][ic];
+            const int sr = t == MTS_NTYPES ? OPP[ic] : MTS_ICSR[t][ic];
+            if (!wl || sr <= 0) continue;
+            spec[t] = (spec[t] & ~(15u << (4 * (sr - 1)))) | ((uint32_t)ic << (4 * (sr - 1)));
+        }
+}
+
 void build_compact(const Geometry& g, CompactImage& ci) {
+    LatticeView v{g.nx, g.ny, g.torus, g.meta.data(), g.pidx.data(), g.gpIn, g.gpOut, nullptr};
+    build_compact(v, ci);
+}
+
+void build_compact(const LatticeView& g, CompactImage& ci) {
     const int nx = g.nx, ny = g.ny;
-    const int64_t P = (int64_t)nx * ny, n = g.ncell;
+    const int64_t P = (int64_t)nx * ny;
+    // cells = the sites carrying FLUID_BIT, in row-major order
+    ci.cidx.assign(P, -1);
+    ci.cellPos.clear();
+    for (int64_t p = 0; p < P; p++)
+        if (g.meta[p] & FLUID_BIT) {
+            ci.cidx[p] = (int32_t)ci.cellPos.size();
+            ci.cellPos.push_back((int32_t)p);
+        }
+    const int64_t n = (int64_t)ci.cellPos.size();
     ci.ncell = n;
     ci.cstride = (n + 255) / 256 * 256;
-    ci.cidx.assign(P, -1);
-    for (int64_t c = 0; c < n; c++) ci.cidx[g.cellPos[c]] = (int32_t)c;
     // rank[p] = cells before site p (row-major)
     std::vector<int32_t> rank(P + 1);
     int32_t acc = 0;
@@ -284,13 +309,13 @@ void build_compact(const Geometry& g, CompactImage& ci) {
     ci.pidx.assign(n, 0);
     ci.outm.assign(n, 0);
     ci.aux.clear();
-    bool specSet[64] = {};
-    for (int t = 0; t < 64; t++) ci.spec[t] = 0;
+    canonical_spec(ci.spec);
     for (int64_t c = 0; c < n; c++) {
-        const int64_t p = g.cellPos[c];
+        const int64_t p = ci.cellPos[c];
         const int x = (int)(p % nx), y = (int)(p / nx);
         const uint32_t w = g.meta[p];
         const uint32_t in = (w >> IN_SHIFT) & 0xffu, out = (w >> OUT_SHIFT) & 0xffu, wall = (w >> WALL_SHIFT) & 0xffu;
+        const int32_t ovr = g.override ? g.override[p] : -1;
         const uint32_t t = (w >> TYPE_SHIFT) & 63u;
         const int yu = wrap(y + 1, ny), yd = wrap(y - 1, ny);
         const int64_t pu = (int64_t)yu * nx + x, pd = (int64_t)yd * nx + x;
@@ -308,15 +333,15 @@ void build_compact(const Geometry& g, CompactImage& ci) {
             spec |= (uint32_t)ic << (4 * (sr - 1));
         }
         // rows 0 and ny-1 of a non-torus lattice lose their y-seam links
-        bool complex = (w & SEAM_BIT) || x == 0 || x == nx - 1 || (!g.torus && (y == 0 || y == ny - 1)) ||
-                       ambiguous || p == g.gpIn || p == g.gpOut || ny < 3;
-        if (!complex && wall) {
-            if (!specSet[t]) {
-                specSet[t] = true;
-                ci.spec[t] = spec;
-            } else if (ci.spec[t] != spec)
-                complex = true;
+        // the kernel takes slot j's specular source from the type table,
+        // kept when that source is one of the cell's wall links
+        uint32_t specT = 0;
+        for (int j = 1; j < 9; j++) {
+            const uint32_t sj = (ci.spec[t] >> (4 * (j - 1))) & 15u;
+            if (sj && ((wall >> (sj - 1)) & 1u)) specT |= sj << (4 * (j - 1));
         }
+        const bool complex = (w & SEAM_BIT) || x == 0 || x == nx - 1 || (!g.torus && (y == 0 || y == ny - 1)) ||
+                             ambiguous || specT != spec || p == g.gpIn || p == g.gpOut || ny < 3 || ovr >= 0;
         auto push_aux = [&]() {  // explicit record of a COMPLEX cell
             const int32_t k = (int32_t)(ci.aux.size() / AUXW);
             for (int j = 1; j < 9; j++) {
@@ -327,7 +352,7 @@ void build_compact(const Geometry& g, CompactImage& ci) {
             ci.aux.push_back(x);
             ci.aux.push_back(y);
             ci.aux.push_back((int32_t)w);
-            ci.aux.push_back(0);
+            ci.aux.push_back(ovr);
             return k;
         };
         ci.pidx[c] = g.pidx[p];

@@ -18,8 +18,12 @@ struct CompactDev {
     const int32_t* cidx = nullptr;
     const int32_t* segs = nullptr;
     const void* cpar = nullptr;  // FP32 parameter records (32 B per (tau_s, Kn) pair)
+    const int32_t* cellPos = nullptr;
     int64_t ncell = 0, cstride = 0;
     int ntx = 0;
+    const float* O = nullptr;  // interface overrides (multi-block buffer cells)
+    int64_t OS = 0;
+    bool ovr = false;
 };
 
 struct LaunchCtx {
@@ -57,6 +61,38 @@ struct ScSpec {
     int* psi_negative = nullptr;
 };
 
+// One level substep of the multi-block FP32 path (cell order).
+struct MbSpec {
+    const float* Ain = nullptr;
+    float* Aout = nullptr;
+    const double* fcan = nullptr;  // mode 0
+    float* m = nullptr;            // stored fields (written for mac 0, read for mac 1)
+    double* usq_old = nullptr;
+};
+// Interface rows of the FP32 path in cell indices of the level images, with
+// the weights formed on the host in FP64 (as the FP64 kernels form them).
+struct CoarseRow32 {
+    int32_t parent, child[4];  // level-2 / level-1 cells (-1 absent)
+    float k;                   // 2 tau_c / mean tau_f
+    int32_t nch;
+    int32_t slot;              // override slot of the parent
+};
+struct FineRow32 {
+    int32_t child[4];  // level-1 cells (fine buffers), override slots 4 r + k
+    int32_t src[6];    // level-2 cells of the interpolation nodes
+    float L[4][6];     // Lagrange weights of the 3 x 2 node grid per child
+    float ratio[4];
+};
+// Interface of the multi-block FP32 path.
+struct IfSpec {
+    const float *A1 = nullptr, *A2 = nullptr;
+    const float *m1 = nullptr, *m2 = nullptr;
+    float *O1 = nullptr, *O2 = nullptr;
+    const CoarseRow32* crows = nullptr;
+    const FineRow32* frows = nullptr;
+    int64_t ncr = 0, nfr = 0;
+};
+
 void upload_icsr(hipStream_t st);
 // FP32 production kernels (mts_fast.inc)
 int launch_transport32(const LaunchCtx& c, int mode, const TransportSpec& s);
@@ -65,6 +101,11 @@ int launch_shanchen32(const LaunchCtx& c, int kind, int mode, const ScSpec& s);
 void upload_spec(const uint32_t* spec64, hipStream_t st);
 int launch_ctransport(const LaunchCtx& c, int mode, const TransportSpec& s);
 int launch_cshanchen(const LaunchCtx& c, int kind, int mode, const ScSpec& s);
+// mode 0 from the canonical state, 1 gather; mac 0 fresh fields, 1 stored
+int launch_cmultistep(const LaunchCtx& c, int mode, int mac, const MbSpec& s);
+int launch_cinterface(const LaunchCtx& c1, const LaunchCtx& c2, const IfSpec& s);
+// first substeps (fresh fields) of two levels in one launch
+int launch_cmultistep2(const LaunchCtx& ca, const MbSpec& sa, const LaunchCtx& cb, const MbSpec& sb, int mode);
 // mode 0 collide-only (fcan -> Aout), 1 fused (Ain -> Aout), 2 stream-only (Ain -> fcan)
 int launch_transport(const LaunchCtx& c, int mode, const TransportSpec& s);
 // kind 0 adsorption, 1 coupled; mode as above

@@ -73,3 +73,32 @@ def test_multiblock_large_torus_runs():
     d1 = g.diagnostics()
     print("mass drift", (d1[0] - d0[0]) / d0[0], "flux", d1[1], "residual", g.residual())
     assert np.isfinite(d1).all() and abs(d1[0] - d0[0]) / d0[0] < 1e-4 and d1[1] > 0
+
+
+def test_multiblock_fp32_reference_seams_against_oracle():
+    banner("multi-block box64 reference seams FP32 vs FP64 oracle 200 steps")
+    pore = box64()
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    o = O.Oracle(pore, kn, lp, multiblock=True, periodic_y=False, Fbody=1e-6)
+    g = Lattice(pore, kn, lp, precision="fp32dev", multiblock=1, periodic_y=0, Fbody=1e-6)
+    o.run("transport", 200, 0.0)
+    g.step("transport", 200)
+    e = {k: rel_l2(g.get(k), o.get(k)) for k in ("rho", "ux", "uy")}
+    print("FP32 multi-block (reference seams) rel-l2", e)
+    assert e["rho"] < 1e-7 and e["ux"] < 2e-3 and e["uy"] < 5e-3
+
+
+def test_multiblock_fp32_call_chunking_is_exact():
+    """A call starts from and ends in the canonical FP64 state (with the
+    interface overrides applied), so 2 x 50 steps equal 100 steps."""
+    banner("multi-block disks128 FP32 100 steps in one call vs two calls")
+    pore = G.disk_packing(128, 0.7, 3, 8, seed=3)
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    a = Lattice(pore, kn, lp, precision="fp32dev", multiblock=1, periodic_y=1, Fbody=1e-6)
+    b = Lattice(pore, kn, lp, precision="fp32dev", multiblock=1, periodic_y=1, Fbody=1e-6)
+    a.step("transport", 100)
+    b.step("transport", 50)
+    b.step("transport", 50)
+    d = np.abs(a.get("f") - b.get("f")).max()
+    print("max |f| diff", d)
+    assert d <= 1e-14
