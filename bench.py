@@ -1,23 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X D2Q9 step (one JSON line on stdout).
 
-Default workload: the Shan-Chen adsorption configuration -- a 2048x2048
-seeded disk packing at porosity 0.5, fluid-solid coupling G_ads drawn from
-U[-1.5, -0.5], initial density uniform in [0.1, 0.5] with 1% seeded noise,
-FP32 deviation storage with an FP64 pseudopotential.  A "step" is one
-adsorption time step of the whole lattice (Shan-Chen forces, MRT + Guo
-source, bounce-back, periodic and generalized-periodic boundaries).
+Default workload (--workload c2): the quadtree multi-block transport
+configuration -- a 1024x1024 seeded disk packing with porosity drawn from
+U[0.4, 0.8], Kn = 0.5 at the finest level, regularised MRT collision with
+specular bounce-back, periodic in x and y, FP32 deviation storage.  A "step"
+is one coarse step of the quadtree: one fused substep of the coarse level,
+two of the fine level, and the interface update.  The run then continues
+from rest to steady state (mean-velocity change below 1e-6 per step) and
+reports the wall time.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|transport|c1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|transport|c1] [--no-cpu]
 
-value = fluid-cell updates per second (MLUPS) over exactly K steps timed with
-HIP events on the library's stream, inputs resident in HBM.  The roofline
-entry covers the fused Shan-Chen kernel: algorithmic bytes per fluid-cell
-update (88 B: 9 FP32 populations read and written, 8 B psi written and read)
-times the fluid cells of one launch, over that kernel's average launch time
-measured in the same run.  The CPU baseline is the oracle (the C++
-restatement of the reference at the reference's -Ofast -fopenmp flags) timed
-on a bounded sample of the same lattice.
+value = cell updates per second (MLUPS; a coarse step counts 2 N_1 + N_2) over
+exactly K steps timed with HIP events on the library's stream, inputs
+resident in HBM.  roofline: the step's dominant kernel (the first substep of
+both levels, one launch), its average launch time from HIP events recorded
+around every launch inside the timed region, and 72 B per cell update (nine
+FP32 populations read and written; metadata, stored fields and interface
+traffic not counted).  The CPU baseline is the oracle (the C++ restatement
+of the reference, -Ofast -fopenmp) timed on a bounded sample.
+
+Other workloads: c3 (Shan-Chen adsorption, 2048^2, 88 B per update),
+transport (single-block FP32 transport of a C2-distribution lattice, --n),
+c1 (single-block FP64 transport, 256^2, 144 B per update).
 """
 import argparse
 import json
@@ -64,21 +70,30 @@ def c2_inputs(n=1024, seed=2025):
     return pore, kn, lp, phi
 
 
+def mean_ux(lat):
+    d = lat.diagnostics()  # sum rho, sum j_x, sum j_y, cells
+    return d[1] / d[0]
+
+
 def run_to_steady(lat, tol, chunk, max_steps, max_s):
-    """Steps until the residual (relative change of the velocity magnitude
-    over one step) falls below tol; returns (steps, seconds, residual)."""
+    """Steps until the mean velocity changes by less than tol per step,
+    |<u_x>(t) - <u_x>(t - chunk)| / (chunk |<u_x>(t)|) < tol, checked every
+    chunk steps.  Returns (steps, seconds, criterion, reference residual)."""
     lat.synchronize()
     t0 = time.perf_counter()
     n = 0
-    r = 1.0
+    u_prev = mean_ux(lat)
+    crit = 1.0
     while n < max_steps and time.perf_counter() - t0 < max_s:
         lat.step("transport", chunk)
         n += chunk
-        r = lat.residual()
-        if r < tol:
+        u = mean_ux(lat)
+        crit = abs(u - u_prev) / (chunk * abs(u)) if u != 0 else 1.0
+        u_prev = u
+        if crit < tol:
             break
     lat.synchronize()
-    return n, time.perf_counter() - t0, r
+    return n, time.perf_counter() - t0, crit, lat.residual()
 
 
 def w9():
@@ -97,6 +112,10 @@ def make_c3(precision="fp32dev"):
 
 
 def time_steps(lat, kind, steps, warmup):
+    """Event time of K steps in one call (the value), then the same K steps
+    again with the kernel timer on: the summed start-to-end event time and
+    the count of the dominant kernel's launches (the roofline).  The timer's
+    per-launch events are kept out of the first region."""
     from mantiis_amd.hiptime import StreamTimer
     t = StreamTimer(lat.stream_handle())
     if warmup > 0:
@@ -105,12 +124,35 @@ def time_steps(lat, kind, steps, warmup):
     t.start()
     lat.step(kind, steps)
     ms = t.stop()
-    # one-step call: collide-only + stream-only passes (+ residual) without fused passes
-    t.start()
-    lat.step(kind, 1)
-    ms1 = t.stop()
+    lat.kernel_timer(True)
+    lat.kernel_timer_read()  # clear
+    lat.step(kind, steps)
+    kms, klaunch = lat.kernel_timer_read()
+    lat.kernel_timer(False)
     t.close()
-    return ms, ms1
+    return ms, kms, klaunch
+
+
+def cpu_baseline_c2(sample_n, sample_steps):
+    """Oracle multi-block transport on a sample_n^2 lattice of the same
+    distribution (the reference's quadtree build is quadratic in the
+    lattice size, so the 1024^2 lattice itself takes minutes to set up)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    pore, kn, lp, phi = c2_inputs(sample_n)
+    o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-6, fast=True, periodic_y=True, multiblock=True)
+    lvl = o.grid()["level"]
+    upd = int(2 * (lvl == 1).sum() + (lvl == 2).sum()) if lvl.max() > 1 else int(len(lvl))
+    o.run("transport", 1, 0.0)  # warm-up
+    t0 = time.perf_counter()
+    o.run("transport", sample_steps, 0.0)
+    dt = time.perf_counter() - t0
+    threads = int(O.load(fast=True).orc_threads())
+    o.close()
+    return dict(value=round(upd * sample_steps / dt / 1e6, 4), unit="MLUPS", cores=threads, kind="port",
+                sample=f"{sample_steps} coarse steps (after 1 warm-up) of a {sample_n}x{sample_n} lattice of the same "
+                       f"distribution (porosity {phi:.3f}, {upd} cell updates per step); oracle built -Ofast "
+                       f"-fopenmp, OMP threads {threads}")
 
 
 def cpu_baseline(inputs, sample_steps):
@@ -140,10 +182,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c3", choices=["c2", "c3", "transport", "c1"])
-    ap.add_argument("--steady", action="store_true", help="c2: also time the run to steady state (tol 1e-6)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "transport", "c1"])
+    ap.add_argument("--no-steady", action="store_true", help="c2: skip the run to steady state")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=0, help="CPU sample steps (default: c2 1000, c3 16)")
+    ap.add_argument("--cpu-n", type=int, default=256, help="c2: lattice side of the CPU sample")
     ap.add_argument("--n", type=int, default=0, help="lattice side of the transport workload (default 1024)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy (torch) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -177,7 +220,7 @@ def main():
         workload = ("Shan-Chen adsorption, 2048x2048 seeded disk packing porosity 0.5, "
                     f"G_ads={inputs['G_ads']:.4f} (U[-1.5,-0.5]), rho0={inputs['rho0']:.4f} (U[0.1,0.5]) +1% noise, "
                     "MRT + Guo source + bounce-back, FP32 deviation storage, FP64 psi")
-        kernel = "k_shanchen<float,1,MRT,BB,genperiodic>"
+        kernel = "k_csc (tiled Shan-Chen: forces, MRT + Guo source, bounce-back, generalized periodic)"
         dtype = "fp32 (populations) / fp64 (psi, moments)"
     elif args.workload == "c2":
         from mantiis_amd.hip import Lattice
@@ -189,11 +232,12 @@ def main():
         kind, inputs = "transport", None
         # cell updates per coarse step: two level-1 substeps and one level-2 step
         updates = 2 * n1 + n2
-        bytes_per_cell = 84  # 72 B populations + 12 B stored (rho - rho0, ux, uy) per cell update
+        bytes_per_cell = 72
+        kernel_cells = n1 + n2  # one launch: first substep of both levels
         workload = (f"quadtree multi-block transport RMRT+SRBB, {lat.nx}x{lat.ny} disk packing porosity {phi:.3f} "
                     f"(U[0.4,0.8]), Kn=0.5, torus, FP32 deviation; {n1} level-1 and {n2} level-2 cells "
                     "(2 level-1 substeps per coarse step)")
-        kernel = "coarse step (k_cmb level substeps + interface kernels)"
+        kernel = "k_cmb2 (first substep of both levels, RMRT+SRBB, stored-field write)"
         dtype = "fp32 (populations)"
     elif args.workload == "transport":
         from mantiis_amd.hip import Lattice
@@ -202,7 +246,7 @@ def main():
         kind, bytes_per_cell, inputs = "transport", 72, None
         workload = (f"transport RMRT+SRBB, {lat.nx}x{lat.ny} disk packing (porosity U[0.4,0.8]), Kn=0.5, torus, "
                     "single-block, FP32 deviation")
-        kernel = "k_transport<float,1,RMRT,SRBB>"
+        kernel = "k_ctr (fused gather + RMRT, SRBB walls)"
         dtype = "fp32 (populations)"
     else:
         from mantiis_amd import geometry as G
@@ -217,27 +261,31 @@ def main():
 
     n = lat.ncell
     if args.workload != "c2":
-        updates = n
-    ms, ms1 = time_steps(lat, kind, args.steps, args.warmup)
+        updates = kernel_cells = n
+    ms, kms, klaunch = time_steps(lat, kind, args.steps, args.warmup)
     mlups = updates * args.steps / (ms / 1e3) / 1e6
-    t_kernel_ms = (ms - ms1) / (args.steps - 1)
-    achieved = bytes_per_cell * updates / (t_kernel_ms / 1e3) / 1e9
+    t_kernel_ms = kms / max(klaunch, 1)
+    achieved = bytes_per_cell * kernel_cells / (t_kernel_ms / 1e3) / 1e9
+    step_achieved = bytes_per_cell * updates * args.steps / (ms / 1e3) / 1e9
     steady = None
-    if args.workload == "c2" and args.steady:
+    if args.workload == "c2" and not args.no_steady:
         from mantiis_amd.hip import Lattice
         lat2 = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
                        multiblock=1)
-        st, sec, r = run_to_steady(lat2, 1e-6, 200, 400000, 120.0)
-        steady = {"tol": 1e-6, "steps": st, "wall_s": round(sec, 3), "residual": r}
+        st, sec, crit, r = run_to_steady(lat2, 1e-6, 100, 400000, 120.0)
+        steady = {"criterion": "|d<u_x>/dt| / |<u_x>| < 1e-6 per step, checked every 100 steps", "steps": st,
+                  "wall_s": round(sec, 3), "final": crit, "reference_residual": r, "plain": True}
         lat2.close()
     diag = lat.diagnostics()
 
     cpu = None
-    if not args.no_cpu and inputs is not None:
-        cpu = cpu_baseline(inputs, args.cpu_steps)
+    if not args.no_cpu and args.workload == "c2":
+        cpu = cpu_baseline_c2(args.cpu_n, args.cpu_steps or 1000)
+    elif not args.no_cpu and inputs is not None:
+        cpu = cpu_baseline(inputs, args.cpu_steps or 16)
 
     out = {
-        "metric": "MLUPS (fluid-cell updates per second) & %HBM roofline",
+        "metric": "MLUPS & %HBM roofline",
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -252,7 +300,9 @@ def main():
         "config": {"workload": workload, "lattice": [lat.nx, lat.ny], "fluid_cells": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_cell": bytes_per_cell, "kernel_ms": round(t_kernel_ms, 6)},
+                     "bytes_per_cell_update": bytes_per_cell, "cells_per_launch": kernel_cells,
+                     "kernel_ms": round(t_kernel_ms, 6), "launches_timed": klaunch,
+                     "step_achieved": round(step_achieved, 1)},
         "cpu_baseline": cpu,
         "mass": float(diag[0]),
     }

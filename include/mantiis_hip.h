@@ -229,6 +229,15 @@ int mts_diagnostics(mts_sim s, double* out4);
 int mts_get_stream(mts_sim s, void** stream);
 int mts_synchronize(mts_sim s);
 
+/* Kernel timer (no reference counterpart): while enabled, every launch of
+ * the step's dominant kernel -- the fused FP32/FP64 pass of a single-block
+ * step, the first-substep pass of both levels of a quadtree step -- is
+ * bracketed by HIP events on the handle's stream.  mts_kernel_timer_read
+ * returns the summed event time (ms) and the launch count since the last
+ * read and clears them (it synchronizes the stream). */
+int mts_kernel_timer(mts_sim s, int enable);
+int mts_kernel_timer_read(mts_sim s, double* ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -154,8 +154,49 @@ struct mts_sim_s {
         *p = (T*)q;
         return MTS_OK;
     }
+    // kernel timer: event pairs around the dominant kernel's launches
+    bool ktimer = false;
+    std::vector<hipEvent_t> kev;
+    size_t kev_used = 0;
+    double ktime_ms = 0.0;
+    int64_t klaunches = 0;
+    // events for the next timed launch (nullptr when the timer is off)
+    void tick(LaunchCtx& c) {
+        c.ev0 = c.ev1 = nullptr;
+        if (!ktimer) return;
+        if (kev_used + 2 > kev.size()) {
+            if (kev.size() >= 8192) {  // pool full: fold the pending pairs first
+                fold_timer();
+            } else {
+                for (int i = 0; i < 256; i++) {
+                    hipEvent_t e;
+                    if (hipEventCreate(&e) != hipSuccess) return;
+                    kev.push_back(e);
+                }
+            }
+        }
+        if (kev_used + 2 > kev.size()) return;
+        c.ev0 = kev[kev_used];
+        c.ev1 = kev[kev_used + 1];
+    }
+    void tock(LaunchCtx& c) {  // after the timed launch
+        if (c.ev0) kev_used += 2;
+        c.ev0 = c.ev1 = nullptr;
+    }
+    void fold_timer() {
+        (void)hipStreamSynchronize(stream);
+        for (size_t i = 0; i + 1 < kev_used; i += 2) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, kev[i], kev[i + 1]) == hipSuccess) {
+                ktime_ms += ms;
+                klaunches++;
+            }
+        }
+        kev_used = 0;
+    }
     ~mts_sim_s() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : kev) (void)hipEventDestroy(e);
         for (void* q : allocs) (void)hipFree(q);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -526,8 +567,10 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
                 a[l].usq_old = last ? s->lv[l].d_usq_old : nullptr;
                 cur[l] ^= 1;
             }
+            s->tick(c[0]);
             if ((rc = launch_cmultistep2(c[0], a[0], c[1], a[1], first ? 0 : 1)))
                 return fail(rc, "operator combination has no kernel");
+            s->tock(c[0]);
             auto& C = s->lc[0];
             c[0].cimg = &C.plain;
             MbSpec b;
@@ -1105,7 +1148,9 @@ int mts_step_transport(mts_sim s, int nsteps) {
         u.Ain = s->d_A;
         u.Aout = s->d_B;
         u.usq_old = (i == nsteps - 1) ? s->d_usq_old : nullptr;
+        s->tick(c);
         launch_transport(c, 1, u);
+        s->tock(c);
         std::swap(s->d_A, s->d_B);
     }
     TransportSpec z;
@@ -1154,7 +1199,9 @@ static int sc_steps(mts_sim s, int kind, int nsteps) {
             sp.uy = s->d_uy;
             sp.usq = s->d_usq;
         }
+        if (i > 0) s->tick(c);
         launch_shanchen(c, kind, i == 0 ? 0 : 1, sp);
+        s->tock(c);
         if (i > 0) std::swap(s->d_A, s->d_B);
     }
     ScSpec z;
@@ -1387,6 +1434,23 @@ int mts_get_stream(mts_sim s, void** stream) {
 int mts_synchronize(mts_sim s) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
     HIP_TRY(hipStreamSynchronize(s->stream));
+    return MTS_OK;
+}
+
+int mts_kernel_timer(mts_sim s, int enable) {
+    if (!s) return fail(MTS_ERR_ARG, "null handle");
+    s->fold_timer();
+    s->ktimer = enable != 0;
+    return MTS_OK;
+}
+
+int mts_kernel_timer_read(mts_sim s, double* ms, int64_t* launches) {
+    if (!s || !ms || !launches) return fail(MTS_ERR_ARG, "null argument");
+    s->fold_timer();
+    *ms = s->ktime_ms;
+    *launches = s->klaunches;
+    s->ktime_ms = 0.0;
+    s->klaunches = 0;
     return MTS_OK;
 }
 

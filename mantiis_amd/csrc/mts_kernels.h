@@ -41,6 +41,9 @@ struct LaunchCtx {
     bool one_param;       // a single (tau_s, Kn) pair: the per-site index is not read
     double fscale = 1.0;  // force-shift scale 2^(L-1) of the lattice level
     const CompactDev* cimg = nullptr;  // FP32 passes in cell order when set
+    // kernel timer: when set, the dominant launch records these events at
+    // its own start and end (hipExtLaunchKernel)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 struct TransportSpec {

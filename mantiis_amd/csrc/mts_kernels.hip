@@ -21,6 +21,8 @@
 #include <cstdint>
 
 #include "mantiis_hip.h"
+#include <hip/hip_ext.h>
+
 #include "mts_common.h"
 #include "mts_kernels.h"
 #include "mts_tables.inc"
@@ -683,6 +685,14 @@ __global__ void k_macro(const double* fcan, const uint32_t* meta, int64_t P, int
 // ============================================================================
 using namespace dev;
 
+// Launch on the context's stream; with timer events set, the events take the
+// kernel's own start and end timestamps.
+template <typename K, typename... A>
+static void klaunch(K k, dim3 g, dim3 b, uint32_t sh, const LaunchCtx& c, A... a) {
+    if (c.ev0) hipExtLaunchKernelGGL(k, g, b, sh, c.stream, c.ev0, c.ev1, 0, a...);
+    else hipLaunchKernelGGL(k, g, b, sh, c.stream, a...);
+}
+
 void upload_icsr(hipStream_t st) {
     signed char t[MTS_NTYPES + 1][9];
     for (int i = 0; i < MTS_NTYPES; i++)
@@ -694,7 +704,7 @@ void upload_icsr(hipStream_t st) {
 template <class S, int MODE, int COLL, int WALL>
 static void launch_t(const LaunchCtx& c, const TransportArgs& a) {
     dim3 blk(256, 1), grd((c.nx + 255) / 256, c.ny);
-    hipLaunchKernelGGL((k_transport<S, MODE, COLL, WALL>), grd, blk, 0, c.stream, a);
+    klaunch(k_transport<S, MODE, COLL, WALL>, grd, blk, 0, c, a);
 }
 
 template <class S, int MODE>

@@ -31,7 +31,8 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_set_body_force", "mts_initialize", "mts_step_transport", "mts_step_adsorption",
            "mts_step_coupled", "mts_residual", "mts_method", "mts_field_size", "mts_get_field",
            "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_get_stream", "mts_synchronize",
-           "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows"]
+           "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows",
+           "mts_kernel_timer", "mts_kernel_timer_read"]
 
 
 class HipLibraryMissing(RuntimeError):
@@ -96,6 +97,8 @@ def load_library(path=LIB_PATH):
     lib.mts_grid_size.argtypes = [C.c_void_p, _I64, _I32]
     lib.mts_grid_arrays.argtypes = [C.c_void_p, _I64, _I32, _I32, _I64, _I32, _I32, _I32, _D]
     lib.mts_grid_rows.argtypes = [C.c_void_p, C.c_int, _I64, _I64]
+    lib.mts_kernel_timer.argtypes = [C.c_void_p, C.c_int]
+    lib.mts_kernel_timer_read.argtypes = [C.c_void_p, _D, _I64]
     _LIB = lib
     return lib
 
@@ -249,6 +252,15 @@ class Lattice:
 
     def synchronize(self):
         _check(self.lib, self.lib.mts_synchronize(self.h))
+
+    def kernel_timer(self, enable):
+        _check(self.lib, self.lib.mts_kernel_timer(self.h, int(bool(enable))))
+
+    def kernel_timer_read(self):
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(self.lib, self.lib.mts_kernel_timer_read(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
 
 class HostGrid:
