@@ -203,14 +203,18 @@ def main():
         return 2
 
     if args.copy_probe:
+        # the library's 4-byte-per-lane copy: reads 1 GiB, writes 1 GiB per launch
         import torch
+        from mantiis_amd.hip import load_library
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         a.fill_(1.0)
         torch.cuda.synchronize()
-        for _ in range(5):
-            torch.mul(a, 1.0, out=b)  # an elementwise shader kernel: reads 1 GiB, writes 1 GiB
-        torch.cuda.synchronize()
+        lib = load_library()
+        rc = lib.mts_probe_copy(b.data_ptr(), a.data_ptr(), a.numel(), 5)
+        if rc != 0:
+            print("probe copy failed:", lib.mts_last_error().decode(), file=sys.stderr)
+            return 1
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "launches": 5}))
         return 0
     if args.workload == "c3":

@@ -238,6 +238,13 @@ int mts_synchronize(mts_sim s);
 int mts_kernel_timer(mts_sim s, int enable);
 int mts_kernel_timer_read(mts_sim s, double* ms, int64_t* launches);
 
+/* Calibration copy (no reference counterpart): copies n FP32 values from
+ * src to dst (device pointers) with one 4-byte nontemporal load and store
+ * per lane, the access width of the lattice kernels, `launches` times on the
+ * default stream, then synchronizes.  Used to scale the rocprofv3
+ * FETCH_SIZE / WRITE_SIZE counters. */
+int mts_probe_copy(float* dst, const float* src, int64_t n, int launches);
+
 #ifdef __cplusplus
 }
 #endif

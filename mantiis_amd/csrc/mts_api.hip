@@ -1437,6 +1437,15 @@ int mts_synchronize(mts_sim s) {
     return MTS_OK;
 }
 
+int mts_probe_copy(float* dst, const float* src, int64_t n, int launches) {
+    if (!dst || !src || n <= 0 || n > ((int64_t)1 << 38) || launches <= 0) return fail(MTS_ERR_ARG, "bad argument");
+    for (int i = 0; i < launches; i++) launch_probe_copy(dst, src, n, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("probe copy: ") + hipGetErrorString(e));
+    return MTS_OK;
+}
+
 int mts_kernel_timer(mts_sim s, int enable) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
     s->fold_timer();

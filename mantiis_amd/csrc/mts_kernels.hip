@@ -814,6 +814,17 @@ void launch_reduce(const LaunchCtx& c, int kind, const double* a0, const double*
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream, kind, nparts, part, out);
 }
 
+namespace dev {
+__global__ void __launch_bounds__(256) k_probe_copy(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+}  // namespace dev
+
+void launch_probe_copy(float* dst, const float* src, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(dev::k_probe_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n);
+}
+
 void launch_gather(const LaunchCtx& c, const double* dense, int q, double* compact) {
     hipLaunchKernelGGL(k_gather, dim3((c.ncell + 255) / 256), dim3(256), 0, c.stream, dense, c.stride, q, c.cellPos,
                        c.ncell, compact);

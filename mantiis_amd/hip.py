@@ -32,7 +32,7 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_step_coupled", "mts_residual", "mts_method", "mts_field_size", "mts_get_field",
            "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_get_stream", "mts_synchronize",
            "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows",
-           "mts_kernel_timer", "mts_kernel_timer_read"]
+           "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy"]
 
 
 class HipLibraryMissing(RuntimeError):
@@ -99,6 +99,7 @@ def load_library(path=LIB_PATH):
     lib.mts_grid_rows.argtypes = [C.c_void_p, C.c_int, _I64, _I64]
     lib.mts_kernel_timer.argtypes = [C.c_void_p, C.c_int]
     lib.mts_kernel_timer_read.argtypes = [C.c_void_p, _D, _I64]
+    lib.mts_probe_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
     _LIB = lib
     return lib
 
