@@ -238,6 +238,38 @@ int mts_synchronize(mts_sim s);
 int mts_kernel_timer(mts_sim s, int enable);
 int mts_kernel_timer_read(mts_sim s, double* ms, int64_t* launches);
 
+/* ------------------------------------------------------------------------
+ * Batched solver (no reference counterpart): nb independent single-block
+ * lattices (torus seams, FP32 deviations, the operators of p) stepped
+ * together, one launch per step over a device work queue of the blocks of
+ * the lattices that have not converged.  Lattice g's inputs are the row-major
+ * nx[g]*ny[g] arrays pore[g], kn[g], locpore[g].  Creation initialises every
+ * lattice at rest (f = w rho_in).
+ * ---------------------------------------------------------------------- */
+typedef struct mts_batch_s* mts_batch;
+int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const* pore, const double* const* kn,
+                     const double* const* locpore, const mts_params* p, mts_batch* out);
+int mts_batch_destroy(mts_batch b);
+/* fluid cells of every lattice (nb entries) */
+int mts_batch_cells(mts_batch b, int64_t* cells);
+/* Steps the active lattices in chunks of check_every (even) steps until each
+ * one's mean velocity <u_x> = sum(rho u_x) / sum(rho) changes by less than
+ * tol per step between two checks, or max_steps have run.  Per lattice:
+ * steps to convergence (negative: -steps run, not converged), seconds since
+ * the call began, and the last <u_x>; cell_updates = cell updates done.
+ * use_graph captures a chunk in a hipGraph and replays it. */
+int mts_batch_run(mts_batch b, int64_t max_steps, int check_every, double tol, int use_graph, int64_t* steps,
+                  double* seconds, double* ux_mean, int64_t* cell_updates);
+/* marks lattices active (1) or not (0) for the next runs */
+int mts_batch_activate(mts_batch b, const int32_t* active);
+/* rho, u_x, u_y (nx*ny row-major, solids 0) of lattice g's post-stream
+ * state at the last check */
+int mts_batch_fields(mts_batch b, int g, double* rho, double* ux, double* uy);
+/* sets lattice g's post-stream populations (9 planes of nx*ny doubles,
+ * f[j*nx*ny + site]; solids ignored): a warm start */
+int mts_batch_set_state(mts_batch b, int g, const double* f);
+int mts_batch_stream(mts_batch b, void** stream);
+
 /* Calibration copy (no reference counterpart): copies n FP32 values from
  * src to dst (device pointers) with one 4-byte nontemporal load and store
  * per lane, the access width of the lattice kernels, `launches` times on the

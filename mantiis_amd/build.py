@@ -33,15 +33,24 @@ def _run(cmd, verbose):
 def build_lib(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     subprocess.run([sys.executable, os.path.join(CSRC, "gen_tables.py")], check=True)
-    objs = []
+    cmds, objs = [], []
     for src in ("mts_grid.cpp", "mts_multigrid.cpp"):
         o = os.path.join(BUILD, src.replace(".cpp", ".o"))
-        _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", o], verbose)
+        cmds.append(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    for src in ("mts_kernels.hip", "mts_api.hip"):
+    for src in ("mts_kernels.hip", "mts_api.hip", "mts_batch.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o], verbose)
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
+    # the translation units are independent: compile them in parallel
+    procs = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    failed = [cmd for cmd, p in zip(cmds, procs) if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)

@@ -29,6 +29,11 @@ static int fail(int code, const std::string& msg) {
     return code;
 }
 
+namespace mts {
+// error reporting shared with the batched solver (mts_batch.hip)
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace mts
+
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
         hipError_t e_ = (expr);                                                              \

@@ -1,0 +1,474 @@
+// Batched solver (C ABI mts_batch_*): many independent single-block
+// lattices in one cell space, stepped by one launch per step over a device
+// work queue of the blocks of the unconverged lattices, with per-lattice
+// convergence on the mean velocity and hipGraph-captured step chunks.
+// The per-lattice model is the single-block FP32 transport step of
+// mts_api.hip (torus seams).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mantiis_hip.h"
+#include "mts_common.h"
+#include "mts_kernels.h"
+
+using namespace mts;
+
+namespace mts {
+int set_error(int code, const std::string& msg);  // mts_api.hip
+}
+
+#define BT_TRY(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_error(e_ == hipErrorOutOfMemory ? MTS_ERR_OOM : MTS_ERR_HIP,        \
+                             std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+struct mts_batch_s {
+    mts_params prm{};
+    Physics ph;
+    hipStream_t stream = nullptr;
+    int nb = 0;
+    std::vector<int> nx, ny;
+    std::vector<int64_t> off, ncell;           // cell range of each lattice
+    std::vector<std::vector<int32_t>> cellPos;  // per lattice: cell -> site
+    std::vector<CellParam> params;
+    CompactImage cat;                          // concatenated image (host)
+    std::vector<float> cpar_h;
+    CompactDev dev;
+    std::vector<BatBlock> hblocks;
+    std::vector<std::vector<int32_t>> blocks_of;  // per lattice: its block ids
+    float* A[2] = {nullptr, nullptr};
+    int cur = 0;
+    float* mac = nullptr;
+    double* acc = nullptr;
+    double* part = nullptr;      // per block
+    int32_t* d_first = nullptr;  // per lattice: first block id (nb + 1)
+    BatBlock* d_blocks = nullptr;
+    int32_t* d_queue = nullptr;
+    int32_t* d_nactive = nullptr;
+    int nactive = 0;
+    std::vector<char> active;
+    std::vector<double> u_prev;
+    std::vector<void*> allocs;
+    // graph of one chunk of steps (replayed while the chunk length holds)
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int gchunk = 0;
+    int gcur = -1;
+
+    template <class T>
+    int alloc(T** p, size_t n) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return set_error(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return MTS_OK;
+    }
+    ~mts_batch_s() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* q : allocs) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    LaunchCtx ctx() const {
+        LaunchCtx c{};
+        c.stream = stream;
+        c.nx = 0;
+        c.ny = 0;
+        c.stride = 0;
+        c.ncell = cat.ncell;
+        c.params = nullptr;
+        c.fp32 = true;
+        c.coll = prm.collision;
+        c.wall = prm.wall;
+        c.rho0 = ph.rhoin;
+        c.rhoin = ph.rhoin;
+        c.Fbody = prm.Fbody;
+        c.one_param = params.size() == 1;
+        c.cimg = &dev;
+        return c;
+    }
+    int upload_queue() {
+        std::vector<int32_t> q;
+        for (int g = 0; g < nb; g++)
+            if (active[g]) q.insert(q.end(), blocks_of[g].begin(), blocks_of[g].end());
+        nactive = (int)q.size();
+        if (!q.empty()) BT_TRY(hipMemcpyAsync(d_queue, q.data(), q.size() * 4, hipMemcpyHostToDevice, stream));
+        BT_TRY(hipMemcpyAsync(d_nactive, &nactive, 4, hipMemcpyHostToDevice, stream));
+        return hipStreamSynchronize(stream) == hipSuccess ? MTS_OK : set_error(MTS_ERR_HIP, "queue upload");
+    }
+    // one step over the queue: A[cur] -> A[cur^1]
+    int step(bool stats, int from) {
+        BatchLaunch L;
+        L.c = ctx();
+        L.Ain = A[cur];
+        L.Aout = A[cur ^ 1];
+        L.blocks = d_blocks;
+        L.queue = d_queue;
+        L.nactive = d_nactive;
+        L.maxblocks = (int)hblocks.size();
+        L.mac = stats ? mac : nullptr;
+        L.part = stats ? part : nullptr;
+        L.from_rest = from;
+        if (from == 2) {  // the given state sits in A[cur^1]: read it, write A[cur]
+            L.Ain = A[cur ^ 1];
+            L.Aout = A[cur];
+        }
+        const int rc = launch_cbatch(L);
+        if (stats) launch_bsum(part, d_first, acc, nb, stream);
+        if (from != 2) cur ^= 1;
+        return rc;
+    }
+    // n steps (n even), the last with statistics
+    int chunk_eager(int n) {
+        int rc;
+        for (int i = 0; i < n; i++)
+            if ((rc = step(i == n - 1, 0))) return rc;
+        return MTS_OK;
+    }
+    int chunk_graph(int n) {
+        if (!gexec || gchunk != n || gcur != cur) {
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            if (graph) (void)hipGraphDestroy(graph);
+            gexec = nullptr;
+            graph = nullptr;
+            const int c0 = cur;
+            BT_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            const int rc = chunk_eager(n);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return set_error(MTS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            graph = g;
+            BT_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            gchunk = n;
+            cur = c0;  // capture did not run anything
+            gcur = c0;
+        }
+        BT_TRY(hipGraphLaunch(gexec, stream));
+        if (n & 1) cur ^= 1;
+        return MTS_OK;
+    }
+};
+
+namespace {
+
+int check(mts_batch b) { return b ? MTS_OK : set_error(MTS_ERR_ARG, "null batch handle"); }
+
+}  // namespace
+
+extern "C" {
+
+int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const* pore, const double* const* kn,
+                     const double* const* locpore, const mts_params* p, mts_batch* out) {
+    if (!out || nb <= 0 || !nx || !ny || !pore || !kn || !locpore) return set_error(MTS_ERR_ARG, "bad argument");
+    *out = nullptr;
+    mts_params prm;
+    if (p) prm = *p;
+    else mts_default_params(&prm);
+    if (prm.precision != MTS_FP32DEV) return set_error(MTS_ERR_UNSUPPORTED, "the batched solver runs FP32 deviations");
+    if (prm.multiblock) return set_error(MTS_ERR_UNSUPPORTED, "the batched solver runs single-block lattices");
+    if (prm.collision < 0 || prm.collision > 2 || prm.wall < 0 || prm.wall > 1 + 1 || prm.wall == MTS_WALL_MDBB)
+        return set_error(MTS_ERR_ARG, "operator combination has no batched kernel");
+    mts_batch_s* b = new (std::nothrow) mts_batch_s();
+    if (!b) return set_error(MTS_ERR_OOM, "host allocation");
+    auto bail = [&](int rc) {
+        delete b;
+        return rc;
+    };
+    b->prm = prm;
+    b->prm.periodic_y = 1;
+    b->ph.Cl = prm.dx;
+    b->ph.Fbody = prm.Fbody;
+    b->ph.T_phy = prm.T;
+    b->ph.P_phy = prm.P;
+    compute_physics(b->ph);
+    if (!b->ph.cubic_ok) return bail(set_error(MTS_ERR_PHYSICS, "PR-EOS: no single positive real root at this (T, P)"));
+    b->nb = nb;
+    b->nx.assign(nx, nx + nb);
+    b->ny.assign(ny, ny + nb);
+    b->off.resize(nb);
+    b->ncell.resize(nb);
+    b->cellPos.resize(nb);
+    b->blocks_of.resize(nb);
+    CompactImage& C = b->cat;
+    int64_t base = 0;
+    for (int g = 0; g < nb; g++) {
+        if (nx[g] <= 0 || ny[g] <= 0 || !pore[g] || !kn[g] || !locpore[g])
+            return bail(set_error(MTS_ERR_ARG, "lattice " + std::to_string(g) + ": bad geometry"));
+        Geometry geo;
+        std::string err;
+        int rc = build_single_grid(nx[g], ny[g], pore[g], kn[g], locpore[g], b->ph, true, geo, err);
+        if (rc) return bail(set_error(rc, "lattice " + std::to_string(g) + ": " + err));
+        CompactImage ci;
+        build_compact(geo, ci);
+        const int32_t poff = (int32_t)b->params.size();
+        if (b->params.size() + geo.params.size() > 65536)
+            return bail(set_error(MTS_ERR_UNSUPPORTED, "more than 65536 distinct (tau_s, Kn) pairs in the batch"));
+        b->params.insert(b->params.end(), geo.params.begin(), geo.params.end());
+        const int64_t n = ci.ncell, naux0 = (int64_t)C.aux.size() / AUXW;
+        if (base + ((n + 255) / 256) * 256 > ((int64_t)1 << 31) - 256)
+            return bail(set_error(MTS_ERR_UNSUPPORTED, "more than 2^31 cells in the batch"));
+        b->off[g] = base;
+        b->ncell[g] = n;
+        b->cellPos[g] = ci.cellPos;
+        C.cw.resize(base + n);
+        C.rr.resize(2 * (base + n));
+        C.pidx.resize(base + n);
+        C.outm.resize(base + n);
+        for (int64_t c = 0; c < n; c++) {
+            const uint32_t cw = ci.cw[c];
+            C.cw[base + c] = cw;
+            C.pidx[base + c] = (uint16_t)(ci.pidx[c] + poff);
+            C.outm[base + c] = ci.outm[c];
+            if (cw & CW_COMPLEX) {
+                C.rr[2 * (base + c)] = (int32_t)(ci.rr[2 * c] + naux0);
+                C.rr[2 * (base + c) + 1] = 0;
+            } else {
+                C.rr[2 * (base + c)] = (int32_t)(ci.rr[2 * c] + base);
+                C.rr[2 * (base + c) + 1] = (int32_t)(ci.rr[2 * c + 1] + base);
+            }
+        }
+        for (int64_t k = 0; k < ci.naux; k++) {
+            const int32_t* r = &ci.aux[k * AUXW];
+            for (int j = 0; j < 8; j++) C.aux.push_back(r[j] >= 0 ? (int32_t)(r[j] + base) : -1);
+            for (int j = 8; j < AUXW; j++) C.aux.push_back(r[j]);
+        }
+        for (int64_t s0 = 0; s0 < n; s0 += 256) {
+            b->blocks_of[g].push_back((int32_t)b->hblocks.size());
+            b->hblocks.push_back(BatBlock{(int32_t)(base + s0), (int32_t)(base + std::min<int64_t>(s0 + 256, n)), g, 0});
+        }
+        base += ((n + 255) / 256) * 256;
+        // padding cells between lattices are never in a block
+        C.cw.resize(base, 0);
+        C.rr.resize(2 * base, 0);
+        C.pidx.resize(base, 0);
+        C.outm.resize(base, 0);
+        if (g == 0) canonical_spec(C.spec);
+    }
+    C.ncell = base;
+    C.cstride = base;
+    C.naux = (int64_t)C.aux.size() / AUXW;
+    C.cellPos.assign(1, 0);
+    C.cidx.assign(1, -1);
+    C.segs.assign(1, 0);
+    C.ntx = 0;
+    hipError_t e = hipSetDevice(prm.device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(set_error(MTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e)));
+    int rc;
+    uint32_t* cw;
+    int32_t *rr, *aux;
+    uint16_t* pidx;
+    uint8_t* outm;
+    float* cpar;
+    b->cpar_h.assign(b->params.size() * 8, 0.0f);
+    for (size_t i = 0; i < b->params.size(); i++) {
+        const CellParam& q = b->params[i];
+        float* o = &b->cpar_h[i * 8];
+        o[0] = (float)q.s_s;
+        o[1] = (float)q.s_q;
+        o[2] = (float)q.s_d;
+        o[3] = (float)q.r_srbb;
+        o[4] = (float)(q.r_srbb - (double)o[3]);
+    }
+    const size_t nA = (size_t)base * 9;
+    if ((rc = b->alloc(&cw, base)) || (rc = b->alloc(&rr, 2 * base)) || (rc = b->alloc(&pidx, base)) ||
+        (rc = b->alloc(&outm, base)) || (rc = b->alloc(&aux, std::max<size_t>(C.aux.size(), 1))) ||
+        (rc = b->alloc(&cpar, b->cpar_h.size())) || (rc = b->alloc(&b->A[0], nA)) || (rc = b->alloc(&b->A[1], nA)) ||
+        (rc = b->alloc(&b->mac, (size_t)base * 3)) || (rc = b->alloc(&b->acc, 2 * (size_t)nb)) ||
+        (rc = b->alloc(&b->part, 2 * b->hblocks.size())) || (rc = b->alloc(&b->d_first, (size_t)nb + 1)) ||
+        (rc = b->alloc(&b->d_blocks, b->hblocks.size())) || (rc = b->alloc(&b->d_queue, b->hblocks.size())) ||
+        (rc = b->alloc(&b->d_nactive, 1)))
+        return bail(rc);
+    e = hipMemcpyAsync(cw, C.cw.data(), base * 4, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rr, C.rr.data(), base * 8, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pidx, C.pidx.data(), base * 2, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(outm, C.outm.data(), base, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess && !C.aux.empty())
+        e = hipMemcpyAsync(aux, C.aux.data(), C.aux.size() * 4, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(cpar, b->cpar_h.data(), b->cpar_h.size() * 4, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_blocks, b->hblocks.data(), b->hblocks.size() * sizeof(BatBlock), hipMemcpyHostToDevice,
+                           b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->mac, 0, (size_t)base * 3 * 4, b->stream);
+    std::vector<int32_t> first(nb + 1);
+    for (int g = 0; g < nb; g++) first[g] = b->blocks_of[g].empty() ? 0 : b->blocks_of[g][0];
+    first[nb] = (int32_t)b->hblocks.size();
+    for (int g = nb - 1; g >= 0; g--)
+        if (b->blocks_of[g].empty()) first[g] = first[g + 1];
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_first, first.data(), (nb + 1) * 4, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->part, 0, 2 * b->hblocks.size() * 8, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return bail(set_error(MTS_ERR_HIP, std::string("upload: ") + hipGetErrorString(e)));
+    upload_spec(C.spec, b->stream);
+    upload_icsr(b->stream);
+    b->dev.cw = cw;
+    b->dev.rr = rr;
+    b->dev.pidx = pidx;
+    b->dev.outm = outm;
+    b->dev.aux = aux;
+    b->dev.cpar = cpar;
+    b->dev.ncell = base;
+    b->dev.cstride = base;
+    // drop the host copies of what now lives on the device
+    C.cw = std::vector<uint32_t>();
+    C.rr = std::vector<int32_t>();
+    C.pidx = std::vector<uint16_t>();
+    C.outm = std::vector<uint8_t>();
+    C.aux = std::vector<int32_t>();
+    b->active.assign(nb, 1);
+    b->u_prev.assign(nb, 0.0);
+    if ((rc = b->upload_queue())) return bail(rc);
+    if ((rc = b->step(true, 1))) return bail(rc);  // rest equilibrium -> first post-collision state
+    e = hipStreamSynchronize(b->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return bail(set_error(MTS_ERR_HIP, std::string("init: ") + hipGetErrorString(e)));
+    *out = b;
+    return MTS_OK;
+}
+
+int mts_batch_destroy(mts_batch b) {
+    delete b;
+    return MTS_OK;
+}
+
+int mts_batch_cells(mts_batch b, int64_t* cells) {
+    if (int rc = check(b)) return rc;
+    if (!cells) return set_error(MTS_ERR_ARG, "null argument");
+    std::copy(b->ncell.begin(), b->ncell.end(), cells);
+    return MTS_OK;
+}
+
+int mts_batch_run(mts_batch b, int64_t max_steps, int check_every, double tol, int use_graph, int64_t* steps,
+                  double* seconds, double* ux_mean, int64_t* cell_updates) {
+    if (int rc = check(b)) return rc;
+    if (max_steps <= 0 || check_every < 2 || check_every % 2 || !(tol > 0))
+        return set_error(MTS_ERR_ARG, "max_steps > 0, even check_every >= 2 and tol > 0 are required");
+    const int nb = b->nb;
+    std::vector<int64_t> st(nb, -1);
+    std::vector<double> sec(nb, -1.0), um(nb, 0.0), hacc(2 * nb);
+    int64_t upd = 0, done = 0;
+    int rc;
+    // baseline mean velocity of the current state
+    for (int g = 0; g < nb; g++)
+        if (!b->active[g]) st[g] = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    bool first = true;
+    while (done < max_steps && b->nactive > 0) {
+        const int n = check_every;
+        if ((rc = use_graph ? b->chunk_graph(n) : b->chunk_eager(n))) return rc;
+        BT_TRY(hipMemcpyAsync(hacc.data(), b->acc, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        BT_TRY(hipStreamSynchronize(b->stream));
+        done += n;
+        int64_t act_cells = 0;
+        for (int g = 0; g < nb; g++)
+            if (b->active[g]) act_cells += b->ncell[g];
+        upd += act_cells * n;
+        const double tnow = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        bool changed = false;
+        for (int g = 0; g < nb; g++) {
+            if (!b->active[g]) continue;
+            const double u = hacc[2 * g + 1] / hacc[2 * g];
+            um[g] = u;
+            if (!std::isfinite(u)) return set_error(MTS_ERR_NONFINITE, "lattice " + std::to_string(g) + ": non-finite");
+            const double crit = u != 0.0 ? std::fabs(u - b->u_prev[g]) / (n * std::fabs(u)) : 1.0;
+            b->u_prev[g] = u;
+            if (!first && crit < tol) {
+                b->active[g] = 0;
+                st[g] = done;
+                sec[g] = tnow;
+                changed = true;
+            }
+        }
+        first = false;
+        if (changed && (rc = b->upload_queue())) return rc;
+    }
+    for (int g = 0; g < nb; g++) {
+        if (st[g] < 0) st[g] = -done;  // not converged within max_steps
+        if (steps) steps[g] = st[g];
+        if (seconds) seconds[g] = sec[g];
+        if (ux_mean) ux_mean[g] = b->active[g] || st[g] > 0 ? um[g] : b->u_prev[g];
+    }
+    if (cell_updates) *cell_updates = upd;
+    return MTS_OK;
+}
+
+int mts_batch_activate(mts_batch b, const int32_t* active) {
+    if (int rc = check(b)) return rc;
+    if (!active) return set_error(MTS_ERR_ARG, "null argument");
+    for (int g = 0; g < b->nb; g++) {
+        b->active[g] = active[g] ? 1 : 0;
+        b->u_prev[g] = 0.0;
+    }
+    return b->upload_queue();
+}
+
+int mts_batch_fields(mts_batch b, int g, double* rho, double* ux, double* uy) {
+    if (int rc = check(b)) return rc;
+    if (g < 0 || g >= b->nb || !rho || !ux || !uy) return set_error(MTS_ERR_ARG, "bad argument");
+    const int64_t n = b->ncell[g], o = b->off[g], S = b->dev.cstride;
+    std::vector<float> h(3 * n);
+    for (int k = 0; k < 3; k++)
+        BT_TRY(hipMemcpyAsync(&h[k * n], b->mac + k * S + o, n * 4, hipMemcpyDeviceToHost, b->stream));
+    BT_TRY(hipStreamSynchronize(b->stream));
+    const int64_t P = (int64_t)b->nx[g] * b->ny[g];
+    std::fill(rho, rho + P, 0.0);
+    std::fill(ux, ux + P, 0.0);
+    std::fill(uy, uy + P, 0.0);
+    const auto& cp = b->cellPos[g];
+    for (int64_t c = 0; c < n; c++) {
+        rho[cp[c]] = b->ph.rhoin + (double)h[c];
+        ux[cp[c]] = h[n + c];
+        uy[cp[c]] = h[2 * n + c];
+    }
+    return MTS_OK;
+}
+
+int mts_batch_set_state(mts_batch b, int g, const double* f) {
+    if (int rc = check(b)) return rc;
+    if (g < 0 || g >= b->nb || !f) return set_error(MTS_ERR_ARG, "bad argument");
+    const int64_t n = b->ncell[g], o = b->off[g], S = b->dev.cstride;
+    const int64_t P = (int64_t)b->nx[g] * b->ny[g];
+    static const double w[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+    std::vector<float> h(9 * n);
+    const auto& cp = b->cellPos[g];
+    for (int64_t c = 0; c < n; c++)
+        for (int j = 0; j < 9; j++) h[j * n + c] = (float)(f[(int64_t)j * P + cp[c]] - w[j] * b->ph.rhoin);
+    // the given post-stream state goes to the other buffer; the lattice's
+    // blocks alone then collide it into the carried state
+    for (int j = 0; j < 9; j++)
+        BT_TRY(hipMemcpyAsync(b->A[b->cur ^ 1] + j * S + o, &h[j * n], n * 4, hipMemcpyHostToDevice, b->stream));
+    std::vector<char> keep = b->active;
+    std::fill(b->active.begin(), b->active.end(), 0);
+    b->active[g] = 1;
+    int rc;
+    if ((rc = b->upload_queue())) return rc;
+    if ((rc = b->step(true, 2))) return rc;
+    b->active = keep;
+    b->u_prev[g] = 0.0;
+    if ((rc = b->upload_queue())) return rc;
+    BT_TRY(hipStreamSynchronize(b->stream));
+    return MTS_OK;
+}
+
+int mts_batch_stream(mts_batch b, void** stream) {
+    if (int rc = check(b)) return rc;
+    if (!stream) return set_error(MTS_ERR_ARG, "null argument");
+    *stream = (void*)b->stream;
+    return MTS_OK;
+}
+
+}  // extern "C"

@@ -107,6 +107,26 @@ int launch_cshanchen(const LaunchCtx& c, int kind, int mode, const ScSpec& s);
 // mode 0 from the canonical state, 1 gather; mac 0 fresh fields, 1 stored
 int launch_cmultistep(const LaunchCtx& c, int mode, int mac, const MbSpec& s);
 int launch_cinterface(const LaunchCtx& c1, const LaunchCtx& c2, const IfSpec& s);
+// Batched single-block lattices (cell order): one block of 256 cells of one
+// lattice per queue entry.
+struct BatBlock {
+    int32_t start, end, geom, pad;
+};
+struct BatchLaunch {
+    LaunchCtx c;  // cimg: the concatenated image
+    const float* Ain = nullptr;
+    float* Aout = nullptr;
+    const BatBlock* blocks = nullptr;
+    const int32_t* queue = nullptr;
+    const int32_t* nactive = nullptr;  // device scalar
+    int maxblocks = 0;                 // grid size (blocks beyond *nactive exit)
+    float* mac = nullptr;
+    double* part = nullptr;  // per-block (sum rho, sum j_x)
+    int from_rest = 0;
+};
+int launch_cbatch(const BatchLaunch& L);
+// acc[2g..2g+1] = sum of part over the blocks [first[g], first[g+1])
+void launch_bsum(const double* part, const int32_t* first, double* acc, int nb, hipStream_t st);
 // first substeps (fresh fields) of two levels in one launch
 int launch_cmultistep2(const LaunchCtx& ca, const MbSpec& sa, const LaunchCtx& cb, const MbSpec& sb, int mode);
 // mode 0 collide-only (fcan -> Aout), 1 fused (Ain -> Aout), 2 stream-only (Ain -> fcan)

@@ -32,7 +32,9 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_step_coupled", "mts_residual", "mts_method", "mts_field_size", "mts_get_field",
            "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_get_stream", "mts_synchronize",
            "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows",
-           "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy"]
+           "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy", "mts_batch_create", "mts_batch_destroy",
+           "mts_batch_cells", "mts_batch_run", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
+           "mts_batch_stream"]
 
 
 class HipLibraryMissing(RuntimeError):
@@ -100,6 +102,15 @@ def load_library(path=LIB_PATH):
     lib.mts_kernel_timer.argtypes = [C.c_void_p, C.c_int]
     lib.mts_kernel_timer_read.argtypes = [C.c_void_p, _D, _I64]
     lib.mts_probe_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
+    _PP = C.POINTER(C.c_void_p)
+    lib.mts_batch_create.argtypes = [C.c_int, _I32, _I32, _PP, _PP, _PP, C.POINTER(Params), _PP]
+    lib.mts_batch_destroy.argtypes = [C.c_void_p]
+    lib.mts_batch_cells.argtypes = [C.c_void_p, _I64]
+    lib.mts_batch_run.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_int, _I64, _D, _D, _I64]
+    lib.mts_batch_activate.argtypes = [C.c_void_p, _I32]
+    lib.mts_batch_fields.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
+    lib.mts_batch_set_state.argtypes = [C.c_void_p, C.c_int, _D]
+    lib.mts_batch_stream.argtypes = [C.c_void_p, _PP]
     _LIB = lib
     return lib
 
@@ -312,3 +323,79 @@ class HostGrid:
         out = np.zeros((cnt.value, 5), np.int64)
         _check(self.lib, self.lib.mts_grid_rows(self.h, w, _p(out, _I64), C.byref(cnt)))
         return out
+
+
+class Batch:
+    """Independent single-block lattices stepped together (mts_batch_*).
+    geoms: list of (pore, kn, locpore) 2-D arrays; params as for Lattice
+    (FP32 deviations, torus seams)."""
+
+    def __init__(self, geoms, **params):
+        self.lib = load_library()
+        params.setdefault("precision", "fp32dev")
+        p = default_params(**params)
+        self.shapes = [np.asarray(g[0]).shape for g in geoms]
+        self._keep = []
+        arrs = []
+        for pore, kn, lp in geoms:
+            a = (np.ascontiguousarray(pore, np.int32), np.ascontiguousarray(kn, np.float64),
+                 np.ascontiguousarray(lp, np.float64))
+            self._keep.append(a)
+            arrs.append(a)
+        nb = len(arrs)
+        nx = np.array([a[0].shape[1] for a in arrs], np.int32)
+        ny = np.array([a[0].shape[0] for a in arrs], np.int32)
+        PP = C.c_void_p * nb
+        pp = PP(*[a[0].ctypes.data for a in arrs])
+        kp = PP(*[a[1].ctypes.data for a in arrs])
+        lpp = PP(*[a[2].ctypes.data for a in arrs])
+        h = C.c_void_p()
+        _check(self.lib, self.lib.mts_batch_create(nb, _p(nx, _I32), _p(ny, _I32), pp, kp, lpp, C.byref(p),
+                                                    C.byref(h)))
+        self._keep = None
+        self.h = h
+        self.nb = nb
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mts_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cells(self):
+        out = np.zeros(self.nb, np.int64)
+        _check(self.lib, self.lib.mts_batch_cells(self.h, _p(out, _I64)))
+        return out
+
+    def run(self, max_steps, check_every=100, tol=1e-6, graph=True):
+        steps = np.zeros(self.nb, np.int64)
+        sec = np.zeros(self.nb)
+        ux = np.zeros(self.nb)
+        upd = C.c_int64()
+        _check(self.lib, self.lib.mts_batch_run(self.h, int(max_steps), int(check_every), float(tol), int(bool(graph)),
+                                                _p(steps, _I64), _p(sec, _D), _p(ux, _D), C.byref(upd)))
+        return dict(steps=steps, seconds=sec, ux=ux, updates=upd.value)
+
+    def activate(self, mask):
+        m = np.ascontiguousarray(mask, np.int32)
+        _check(self.lib, self.lib.mts_batch_activate(self.h, _p(m, _I32)))
+
+    def fields(self, g):
+        shp = self.shapes[g]
+        rho, ux, uy = (np.zeros(shp) for _ in range(3))
+        _check(self.lib, self.lib.mts_batch_fields(self.h, int(g), _p(rho, _D), _p(ux, _D), _p(uy, _D)))
+        return rho, ux, uy
+
+    def set_state(self, g, f):
+        f = np.ascontiguousarray(f, np.float64)
+        _check(self.lib, self.lib.mts_batch_set_state(self.h, int(g), _p(f, _D)))
+
+    def stream_handle(self):
+        s = C.c_void_p()
+        _check(self.lib, self.lib.mts_batch_stream(self.h, C.byref(s)))
+        return s.value

@@ -1,0 +1,93 @@
+"""Batched solver (mts_batch_*): independent single-block lattices stepped by
+one launch per step over a work queue of unconverged lattices.  Each lattice
+of a batch must follow exactly the single-block FP32 path of the same
+lattice; graph replays must equal eager launches; converged lattices leave
+the queue and stop changing."""
+import numpy as np
+import pytest
+
+from _gpu_util import banner
+from mantiis_amd import geometry as G
+from mantiis_amd.hip import Batch, Lattice
+
+pytestmark = pytest.mark.gpu
+MFP_DX = 0.7730846760658538
+
+
+def geoms():
+    out = []
+    for n, phi, kn, seed in ((64, 0.7, 0.1, 1), (96, 0.55, 0.03, 2), (128, 0.75, 0.8, 3)):
+        pore = G.disk_packing(n, phi, 3, 9, seed=seed)
+        k, lp = G.uniform_kn(pore, kn, MFP_DX)
+        out.append((pore, k, lp))
+    return out
+
+
+PARAMS = dict(collision=2, wall=2, dx=1e-9, Fbody=1e-6)
+
+
+def test_batch_lattices_equal_single_block():
+    banner("batch of 3 lattices vs single-block FP32 runs, 300 steps")
+    gs = geoms()
+    b = Batch(gs, **PARAMS)
+    r = b.run(300, check_every=300, tol=1e-300, graph=False)
+    assert (r["steps"] == -300).all()
+    for g, (pore, kn, lp) in enumerate(gs):
+        lat = Lattice(pore, kn, lp, precision="fp32dev", periodic_y=1, **PARAMS)
+        lat.step("transport", 300)
+        ux, uy, rho = lat.complete_domain()
+        brho, bux, buy = b.fields(g)
+        fl = pore.reshape(-1) == 1
+        d = max(np.abs(bux.reshape(-1) - ux)[fl].max(), np.abs(buy.reshape(-1) - uy)[fl].max(),
+                np.abs(brho.reshape(-1) - rho)[fl].max())
+        print("lattice", g, "max |diff|", d)
+        assert d == 0.0
+
+
+def test_batch_graph_replays_equal_eager():
+    banner("batch graph replays vs eager launches, 6 chunks of 50 steps")
+    gs = geoms()
+    a = Batch(gs, **PARAMS)
+    b = Batch(gs, **PARAMS)
+    for _ in range(6):
+        ra = a.run(50, check_every=50, tol=1e-300, graph=True)
+        rb = b.run(50, check_every=50, tol=1e-300, graph=False)
+        assert np.array_equal(ra["ux"], rb["ux"])
+        for g in range(len(gs)):
+            fa, fb = a.fields(g), b.fields(g)
+            assert all(np.array_equal(x, y) for x, y in zip(fa, fb))
+
+
+def test_batch_converges_and_freezes_converged_lattices():
+    banner("batch run to 1e-6, converged lattices leave the queue")
+    gs = geoms()
+    b = Batch(gs, **PARAMS)
+    r = b.run(200000, check_every=100, tol=1e-6, graph=True)
+    print("steps", r["steps"], "seconds", r["seconds"], "ux", r["ux"], "updates", r["updates"])
+    assert (r["steps"] > 0).all() and (r["ux"] > 0).all()
+    assert r["updates"] == sum(int(s) * int(n) for s, n in zip(r["steps"], b.cells()))
+    before = [b.fields(g) for g in range(len(gs))]
+    r2 = b.run(400, check_every=100, tol=1e-6)  # nothing is active any more
+    assert r2["updates"] == 0
+    after = [b.fields(g) for g in range(len(gs))]
+    assert all(np.array_equal(x, y) for fa, fb in zip(before, after) for x, y in zip(fa, fb))
+
+
+def test_batch_warm_start_from_converged_state():
+    banner("batch warm start: a converged state converges again at the first check")
+    gs = geoms()[:2]
+    b = Batch(gs, **PARAMS)
+    r = b.run(200000, check_every=100, tol=1e-6)
+    cold = r["steps"].copy()
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    ex = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+    ey = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+    c = Batch(gs, **PARAMS)
+    for g in range(len(gs)):
+        rho, ux, uy = b.fields(g)
+        cu = ex[:, None, None] * ux[None] + ey[:, None, None] * uy[None]
+        f = w[:, None, None] * rho[None] * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (ux ** 2 + uy ** 2)[None])
+        c.set_state(g, f)
+    r = c.run(200000, check_every=100, tol=1e-6)
+    print("cold", cold, "warm", r["steps"])
+    assert (r["steps"] > 0).all() and (r["steps"] < cold).all()
