@@ -75,6 +75,27 @@ def mean_ux(lat):
     return d[1] / d[0]
 
 
+def c4_one(args):
+    """One lattice of the batched configuration: side L in {128, 256, 512,
+    1024}, porosity U[0.4, 0.8], disk radius U[3, 16], Kn log-uniform in
+    [0.01, 1] (LocPore = (mfp/dx)/Kn)."""
+    from mantiis_amd import geometry as G
+    seed, = args
+    rng = np.random.default_rng(seed)
+    L = int(rng.choice([128, 256, 512, 1024]))
+    phi = float(rng.uniform(0.4, 0.8))
+    kn = float(np.exp(rng.uniform(np.log(0.01), np.log(1.0))))
+    pore = G.disk_packing(L, phi, 3, 16, seed=seed)
+    knf, lp = G.uniform_kn(pore, kn, 0.7730846760658538)
+    return pore, knf, lp, kn
+
+
+def c4_inputs(nb=256, seed=4000, workers=8):
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(c4_one, [(seed + i,) for i in range(nb)])
+
+
 def run_to_steady(lat, tol, chunk, max_steps, max_s):
     """Steps until the mean velocity changes by less than tol per step,
     |<u_x>(t) - <u_x>(t - chunk)| / (chunk |<u_x>(t)|) < tol, checked every
@@ -94,6 +115,26 @@ def run_to_steady(lat, tol, chunk, max_steps, max_s):
             break
     lat.synchronize()
     return n, time.perf_counter() - t0, crit, lat.residual()
+
+
+PROFILE_FOR = {"c2": "k_cmb2_quadtree_transport_1024.md", "c3": "k_csc_shanchen_adsorption_2048.md",
+               "transport": "k_ctr_transport_4096.md"}
+
+
+def profiled_traffic(workload, n):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 summary of the same workload (FETCH_SIZE + WRITE_SIZE scaled by
+    the copy-kernel calibration); None when there is none."""
+    name = PROFILE_FOR.get(workload)
+    if name is None or (workload == "transport" and n != 4096):
+        return None
+    try:
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def w9():
@@ -177,12 +218,68 @@ def cpu_baseline(inputs, sample_steps):
                        f"serial as in the reference)")
 
 
+def bench_c4(args):
+    """The batched configuration: every lattice from rest to steady state
+    (mean-velocity change below 1e-6 per step, checked every 100 steps) in
+    one batch; value = cell updates per second over that run."""
+    from mantiis_amd.hip import Batch
+    from mantiis_amd.hiptime import StreamTimer
+    t0 = time.perf_counter()
+    geoms = c4_inputs(args.batch)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b = Batch([(g[0], g[1], g[2]) for g in geoms], collision=2, wall=2, dx=1e-9, Fbody=1e-6)
+    t_setup = time.perf_counter() - t0
+    cells = b.cells()
+    t = StreamTimer(b.stream_handle())
+    t.start()
+    r = b.run(400000, check_every=100, tol=1e-6, graph=True)
+    ms = t.stop()
+    t.close()
+    mlups = r["updates"] / (ms / 1e3) / 1e6
+    achieved = 72 * r["updates"] / (ms / 1e3) / 1e9
+    conv = r["steps"] > 0
+    sizes = np.array([g[0].shape[0] for g in geoms])
+    by_size = {int(L): {"lattices": int((sizes == L).sum()),
+                        "median_steps": int(np.median(np.abs(r["steps"][sizes == L]))) if (sizes == L).any() else 0}
+               for L in (128, 256, 512, 1024)}
+    out = {
+        "metric": "MLUPS & %HBM roofline",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": int(np.abs(r["steps"]).max()),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (populations)",
+        "data": "synthetic (seeded disk packings)",
+        "config": {"workload": f"batched: {args.batch} lattices, side U{{128,256,512,1024}}, porosity U[0.4,0.8], "
+                               "disk radius U[3,16], Kn log-uniform [0.01,1], RMRT+SRBB, torus, each run from rest "
+                               "to |d<u_x>/dt|/|<u_x>| < 1e-6 per step (checked every 100 steps) in one batch",
+                   "lattices": args.batch, "fluid_cells": int(cells.sum()), "parallelism": "single GPU",
+                   "by_side": by_size},
+        "roofline": {"bound": "hbm", "kernel": "k_cbat (one launch per step over the queue of unconverged lattices; "
+                                               "whole run incl. convergence checks)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_cell_update": 72},
+        "cpu_baseline": None,
+        "batch": {"converged": int(conv.sum()), "run_s": round(ms / 1e3, 3), "cell_updates": int(r["updates"]),
+                  "generation_s": round(t_gen, 1), "setup_s": round(t_setup, 1)},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "transport", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "transport", "c1"])
+    ap.add_argument("--batch", type=int, default=256, help="c4: number of lattices")
     ap.add_argument("--no-steady", action="store_true", help="c2: skip the run to steady state")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU sample steps (default: c2 1000, c3 16)")
@@ -217,6 +314,8 @@ def main():
             return 1
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "launches": 5}))
         return 0
+    if args.workload == "c4":
+        return bench_c4(args)
     if args.workload == "c3":
         lat, inputs = make_c3()
         kind = "adsorption"
@@ -271,6 +370,18 @@ def main():
     t_kernel_ms = kms / max(klaunch, 1)
     achieved = bytes_per_cell * kernel_cells / (t_kernel_ms / 1e3) / 1e9
     step_achieved = bytes_per_cell * updates * args.steps / (ms / 1e3) / 1e9
+    dominant = None
+    if args.workload == "c2":
+        # The quadtree step is four short launches (~20 us each), for which
+        # per-launch events add several us; the roofline is taken over the
+        # whole coarse step from the events around the K-step call, and the
+        # dominant launch is reported beside it.
+        dominant = {"kernel": kernel, "event_ms": round(t_kernel_ms, 6), "cells_per_launch": kernel_cells,
+                    "achieved": round(achieved, 1)}
+        kernel = "coarse step: k_cmb2 + k_cmb + k_if32 (coarse rows) + k_if32 (fine rows)"
+        t_kernel_ms = ms / args.steps
+        kernel_cells = updates
+        achieved = step_achieved
     steady = None
     if args.workload == "c2" and not args.no_steady:
         from mantiis_amd.hip import Lattice
@@ -303,7 +414,8 @@ def main():
         "data": "synthetic (seeded disk packing)",
         "config": {"workload": workload, "lattice": [lat.nx, lat.ny], "fluid_cells": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(args.workload, args.n),
                      "bytes_per_cell_update": bytes_per_cell, "cells_per_launch": kernel_cells,
                      "kernel_ms": round(t_kernel_ms, 6), "launches_timed": klaunch,
                      "step_achieved": round(step_achieved, 1)},
@@ -312,6 +424,7 @@ def main():
     }
     if args.workload == "c2":
         out["config"]["cell_updates_per_step"] = updates
+        out["roofline"]["dominant_launch"] = dominant
     if steady is not None:
         out["steady"] = steady
     print(json.dumps(out), flush=True)
