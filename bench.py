@@ -301,18 +301,17 @@ def main():
 
     if args.copy_probe:
         # the library's 4-byte-per-lane copy: reads 1 GiB, writes 1 GiB per launch
-        import torch
-        from mantiis_amd.hip import load_library
-        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(1.0)
-        torch.cuda.synchronize()
+        from mantiis_amd.hip import DeviceArray, load_library
+        n = 1 << 28
+        a = DeviceArray((n,))
+        b = DeviceArray((n,))
+        a.upload(np.ones(n, np.float32))
         lib = load_library()
-        rc = lib.mts_probe_copy(b.data_ptr(), a.data_ptr(), a.numel(), 5)
+        rc = lib.mts_probe_copy(b.ptr, a.ptr, n, 5)
         if rc != 0:
             print("probe copy failed:", lib.mts_last_error().decode(), file=sys.stderr)
             return 1
-        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "launches": 5}))
+        print(json.dumps({"copy_probe_bytes": n * 4, "launches": 5}))
         return 0
     if args.workload == "c4":
         return bench_c4(args)

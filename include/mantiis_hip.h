@@ -270,6 +270,42 @@ int mts_batch_fields(mts_batch b, int g, double* rho, double* ux, double* uy);
 int mts_batch_set_state(mts_batch b, int g, const double* f);
 int mts_batch_stream(mts_batch b, void** stream);
 
+/* ------------------------------------------------------------------------
+ * Learned warm start (no reference counterpart): a four-level convolutional
+ * encoder-decoder (widths c0, 2c0, 4c0, 8c0; 3x3 periodic convolutions, ReLU,
+ * 2x2 average pooling, nearest upsampling with skip concatenation, a 1x1
+ * head) from 3 input channels (fluid mask, log10 Kn, forcing) to 2 output
+ * channels (scaled velocity), trained with its own BF16 matrix-core forward
+ * and backward kernels.  Inputs, targets, masks and outputs are device
+ * pointers, NHWC FP32: in [B][H][W][3], out/target [B][H][W][2], mask
+ * [B][H][W]; H and W multiples of 8 within the sizes given at creation.
+ * ---------------------------------------------------------------------- */
+typedef struct mts_net_s* mts_net;
+int mts_net_create(int c0, int max_batch, int max_h, int max_w, uint64_t seed, mts_net* out);
+int mts_net_destroy(mts_net n);
+int mts_net_param_count(mts_net n, int64_t* count);
+/* parameters and gradients as one FP32 host array: per layer the weights
+ * [3*3 or 1][cout][cin] then the biases [cout] */
+int mts_net_get_params(mts_net n, float* host);
+int mts_net_set_params(mts_net n, const float* host);
+int mts_net_get_grads(mts_net n, float* host);
+int mts_net_forward(mts_net n, const float* in, int B, int H, int W, float* out);
+/* after mts_net_forward: masked mean-square loss sum m |o - t|^2 / (2 sum m)
+ * and the gradient of every parameter */
+int mts_net_backward(mts_net n, const float* target, const float* mask, float* loss);
+int mts_net_adam(mts_net n, float lr, float beta1, float beta2, float eps, int step);
+/* the convolutions' GEMM kernel on dense host matrices (check of the matrix
+ * core lane layout): C[M x N] = A[M x K] B[K x N] with BF16 operands */
+int mts_net_gemm_check(const float* A, const float* B, int M, int N, int K, float* C);
+int mts_net_stream(mts_net n, void** stream);
+
+/* Device memory for the network's tensors and the calibration copy (no
+ * reference counterpart): plain hipMalloc / hipFree / synchronous copies. */
+int mts_dev_alloc(size_t bytes, void** ptr);
+int mts_dev_free(void* ptr);
+int mts_dev_upload(void* dst, const void* src, size_t bytes);
+int mts_dev_download(void* dst, const void* src, size_t bytes);
+
 /* Calibration copy (no reference counterpart): copies n FP32 values from
  * src to dst (device pointers) with one 4-byte nontemporal load and store
  * per lane, the access width of the lattice kernels, `launches` times on the

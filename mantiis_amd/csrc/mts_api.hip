@@ -1442,6 +1442,31 @@ int mts_synchronize(mts_sim s) {
     return MTS_OK;
 }
 
+int mts_dev_alloc(size_t bytes, void** ptr) {
+    if (!ptr || bytes == 0) return fail(MTS_ERR_ARG, "bad argument");
+    *ptr = nullptr;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) return fail(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return MTS_OK;
+}
+
+int mts_dev_free(void* ptr) {
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return MTS_OK;
+}
+
+int mts_dev_upload(void* dst, const void* src, size_t bytes) {
+    if (!dst || !src) return fail(MTS_ERR_ARG, "null argument");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return MTS_OK;
+}
+
+int mts_dev_download(void* dst, const void* src, size_t bytes) {
+    if (!dst || !src) return fail(MTS_ERR_ARG, "null argument");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return MTS_OK;
+}
+
 int mts_probe_copy(float* dst, const float* src, int64_t n, int launches) {
     if (!dst || !src || n <= 0 || n > ((int64_t)1 << 38) || launches <= 0) return fail(MTS_ERR_ARG, "bad argument");
     for (int i = 0; i < launches; i++) launch_probe_copy(dst, src, n, nullptr);

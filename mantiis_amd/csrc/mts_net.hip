@@ -1,0 +1,722 @@
+// Learned warm start: a four-level convolutional encoder-decoder (U-Net)
+// that maps (fluid mask, log10 Kn, forcing) to the steady velocity field of a
+// periodic porous lattice, with its own forward and backward kernels.
+//
+// Every convolution (3x3 and 1x1, periodic padding) is an implicit GEMM on
+// BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulation):
+//   forward        Y[p, co]      = sum_{kk, ci} X[p + o(kk), ci] W[kk, co, ci]
+//   backward data  dX[p, ci]     = sum_{kk, co} dY[p - o(kk), co] W[kk, co, ci]
+//   backward wgt   dW[kk, co, ci] = sum_p dY[p, co] X[p + o(kk), ci]
+// One kernel template runs all three: a 64x64 block tile, K in chunks of 32
+// staged through LDS by per-element gather functors, four wave64s each on a
+// 32x32 sub-tile (2x2 MFMA tiles).  The weight gradient splits K (pixels)
+// over blocks and sums the partials in a fixed order.
+// Activations are NHWC BF16; activation gradients FP32 (converted to BF16,
+// with the ReLU mask applied, before they enter a GEMM); parameters FP32
+// masters with BF16 copies; Adam in FP32.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "mantiis_hip.h"
+
+namespace mts {
+int set_error(int code, const std::string& msg);  // mts_api.hip
+}
+using mts::set_error;
+
+#define NET_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_error(e_ == hipErrorOutOfMemory ? MTS_ERR_OOM : MTS_ERR_HIP,        \
+                             std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+namespace net {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;
+
+__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// ---------------------------------------------------------------- GEMM
+// C[M x N] = A[M x K] B[K x N] over the K range of blockIdx.z (split-K).
+// Lane layout of v_mfma_f32_16x16x32_bf16 (wave64): lane l holds
+// A[l % 16][8 (l / 16) .. +7], B[8 (l / 16) .. +7][l % 16] and
+// D[4 (l / 16) .. +3][l % 16].
+template <class Op>
+__global__ void __launch_bounds__(256) k_gemm(Op op, int M, int N, int K, int nsplit, int kchunk) {
+    __shared__ bf16 sA[TM][LDK];
+    __shared__ bf16 sB[TN][LDK];
+    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN, z = blockIdx.z;
+    const int kb = (z % nsplit) * kchunk, ke = min(K, kb + kchunk);
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int r = t >> 2, kq = (t & 3) * 8;
+    for (int k0 = kb; k0 < ke; k0 += TK) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = k0 + kq + e;
+            const bool kin = k < ke;
+            sA[r][kq + e] = (kin && m0 + r < M) ? op.a(m0 + r, k, z) : (bf16)0.0f;
+            sB[r][kq + e] = (kin && n0 + r < N) ? op.b(k, n0 + r, z) : (bf16)0.0f;
+        }
+        __syncthreads();
+        const int i = l & 15, q = (l >> 4) * 8;
+        bf16x8 af[2], bfm[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) af[a] = *(const bf16x8*)&sA[wm + a * 16 + i][q];
+#pragma unroll
+        for (int b = 0; b < 2; b++) bfm[b] = *(const bf16x8*)&sB[wn + b * 16 + i][q];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfm[b], acc[a][b], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int m = m0 + wm + a * 16 + 4 * (l >> 4) + rr, n = n0 + wn + b * 16 + (l & 15);
+                if (m < M && n < N) op.store(m, n, acc[a][b][rr], z);
+            }
+}
+
+// dense row-major operands (the GEMM check)
+struct OpDense {
+    const bf16 *A, *B;
+    float* C;
+    int K, N;
+    __device__ bf16 a(int m, int k, int) const { return A[(int64_t)m * K + k]; }
+    __device__ bf16 b(int k, int n, int) const { return B[(int64_t)k * N + n]; }
+    __device__ void store(int m, int n, float v, int) const { C[(int64_t)m * N + n] = v; }
+};
+
+struct Geom {
+    int B, H, W;
+    __device__ __forceinline__ void pix(int m, int& b, int& y, int& x) const {
+        x = m % W;
+        const int t = m / W;
+        y = t % H;
+        b = t / H;
+    }
+};
+
+// kernel offsets: kk = 3 (dy + 1) + (dx + 1) for 3x3, kk = 0 for 1x1
+__device__ __forceinline__ void koff(int kk, int ksz, int& dy, int& dx) {
+    dy = ksz == 3 ? kk / 3 - 1 : 0;
+    dx = ksz == 3 ? kk % 3 - 1 : 0;
+}
+
+struct OpConvFwd {
+    Geom g;
+    const bf16* X;   // [p][Cin]
+    const bf16* Wt;  // [kk][Cout][Cin]
+    const float* bias;
+    bf16* Y;         // [p][Cout] (BF16, ReLU) or
+    float* Yf;       // [p][Cout] (FP32, no ReLU)
+    int Cin, Cout, ksz, relu;
+    __device__ bf16 a(int m, int k, int) const {
+        const int kk = k / Cin, ci = k - kk * Cin;
+        int b, y, x, dy, dx;
+        g.pix(m, b, y, x);
+        koff(kk, ksz, dy, dx);
+        const int yy = wrapi(y + dy, g.H), xx = wrapi(x + dx, g.W);
+        return X[(((int64_t)b * g.H + yy) * g.W + xx) * Cin + ci];
+    }
+    __device__ bf16 b(int k, int n, int) const {
+        const int kk = k / Cin, ci = k - kk * Cin;
+        return Wt[((int64_t)kk * Cout + n) * Cin + ci];
+    }
+    __device__ void store(int m, int n, float v, int) const {
+        v += bias[n];
+        if (relu) v = fmaxf(v, 0.0f);
+        if (Yf) Yf[(int64_t)m * Cout + n] = v;
+        else Y[(int64_t)m * Cout + n] = (bf16)v;
+    }
+};
+
+struct OpConvBwdData {
+    Geom g;
+    const bf16* dY;  // [p][Cout]
+    const bf16* Wt;  // [kk][Cout][Cin]
+    float* dX;       // [p][Cin]
+    int Cin, Cout, ksz, add;
+    __device__ bf16 a(int m, int k, int) const {
+        const int kk = k / Cout, co = k - kk * Cout;
+        int b, y, x, dy, dx;
+        g.pix(m, b, y, x);
+        koff(kk, ksz, dy, dx);
+        const int yy = wrapi(y - dy, g.H), xx = wrapi(x - dx, g.W);
+        return dY[(((int64_t)b * g.H + yy) * g.W + xx) * Cout + co];
+    }
+    __device__ bf16 b(int k, int n, int) const {
+        const int kk = k / Cout, co = k - kk * Cout;
+        return Wt[((int64_t)kk * Cout + co) * Cin + n];
+    }
+    __device__ void store(int m, int n, float v, int) const {
+        float* o = dX + (int64_t)m * Cin + n;
+        *o = add ? *o + v : v;
+    }
+};
+
+struct OpConvBwdW {
+    Geom g;
+    const bf16* dY;  // [p][Cout]
+    const bf16* X;   // [p][Cin]
+    float* part;     // [z][Cout][Cin], z = kk * nsplit + s
+    int Cin, Cout, ksz, nsplit;
+    __device__ bf16 a(int m, int k, int) const { return dY[(int64_t)k * Cout + m]; }
+    __device__ bf16 b(int k, int n, int z) const {
+        const int kk = z / nsplit;
+        int b, y, x, dy, dx;
+        g.pix(k, b, y, x);
+        koff(kk, ksz, dy, dx);
+        const int yy = wrapi(y + dy, g.H), xx = wrapi(x + dx, g.W);
+        return X[(((int64_t)b * g.H + yy) * g.W + xx) * Cin + n];
+    }
+    __device__ void store(int m, int n, float v, int z) const { part[((int64_t)z * Cout + m) * Cin + n] = v; }
+};
+
+// ---------------------------------------------------------------- small kernels
+__global__ void k_sum_parts(const float* part, float* dW, int nkk, int nsplit, int64_t per) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nkk * per) return;
+    const int64_t kk = i / per, e = i - kk * per;
+    float s = 0.0f;
+    for (int q = 0; q < nsplit; q++) s += part[((int64_t)kk * nsplit + q) * per + e];
+    dW[i] = s;
+}
+
+// bias gradient: db[c] = sum_p g[p][c] (one block per channel, fixed order)
+__global__ void __launch_bounds__(256) k_bias_grad(const bf16* g, int64_t P, int C, float* db) {
+    __shared__ float s[256];
+    const int c = blockIdx.x, t = threadIdx.x;
+    float v = 0.0f;
+    for (int64_t p = t; p < P; p += 256) v += (float)g[p * C + c];
+    s[t] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) s[t] += s[t + h];
+        __syncthreads();
+    }
+    if (t == 0) db[c] = s[0];
+}
+
+// g (FP32) -> BF16 operand, times the ReLU mask of the layer output y
+__global__ void k_grad_prep(const float* g, const bf16* y, bf16* out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (bf16)((y && (float)y[i] <= 0.0f) ? 0.0f : g[i]);
+}
+
+__global__ void k_f2bf(const float* in, bf16* out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (bf16)in[i];
+}
+
+// 2x2 average pool (NHWC): out [B][H/2][W/2][C]
+__global__ void k_pool(const bf16* in, bf16* out, int B, int H, int W, int C) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = H / 2, w = W / 2;
+    if (i >= (int64_t)B * h * w * C) return;
+    const int c = (int)(i % C);
+    int64_t t = i / C;
+    const int x = (int)(t % w);
+    t /= w;
+    const int y = (int)(t % h);
+    const int b = (int)(t / h);
+    auto at = [&](int yy, int xx) { return (float)in[(((int64_t)b * H + yy) * W + xx) * C + c]; };
+    out[i] = (bf16)(0.25f * (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1) + at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1)));
+}
+// its gradient, added into gin [B][H][W][C]
+__global__ void k_pool_bwd(const float* gout, float* gin, int B, int H, int W, int C) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * H * W * C) return;
+    const int c = (int)(i % C);
+    int64_t t = i / C;
+    const int x = (int)(t % W);
+    t /= W;
+    const int y = (int)(t % H);
+    const int b = (int)(t / H);
+    gin[i] += 0.25f * gout[(((int64_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+}
+
+// nearest 2x upsample of lo [B][H/2][W/2][C1] next to skip [B][H][W][C2]:
+// out [B][H][W][C1 + C2]
+__global__ void k_upcat(const bf16* lo, int C1, const bf16* skip, int C2, bf16* out, int B, int H, int W) {
+    const int C = C1 + C2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * H * W * C) return;
+    const int c = (int)(i % C);
+    const int64_t p = i / C;
+    const int x = (int)(p % W);
+    const int64_t t = p / W;
+    const int y = (int)(t % H);
+    const int b = (int)(t / H);
+    out[i] = c < C1 ? lo[(((int64_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C1 + c] : skip[p * C2 + (c - C1)];
+}
+// its gradient: glo [B][H/2][W/2][C1] = sum of the four children (overwritten),
+// gskip [B][H][W][C2] = the skip part (overwritten)
+__global__ void k_upcat_bwd_lo(const float* g, int C1, int C2, float* glo, int B, int H, int W) {
+    const int C = C1 + C2, h = H / 2, w = W / 2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * h * w * C1) return;
+    const int c = (int)(i % C1);
+    int64_t t = i / C1;
+    const int x = (int)(t % w);
+    t /= w;
+    const int y = (int)(t % h);
+    const int b = (int)(t / h);
+    float s = 0.0f;
+    for (int dy = 0; dy < 2; dy++)
+        for (int dx = 0; dx < 2; dx++) s += g[(((int64_t)b * H + 2 * y + dy) * W + 2 * x + dx) * C + c];
+    glo[i] = s;
+}
+__global__ void k_upcat_bwd_skip(const float* g, int C1, int C2, float* gskip, int64_t P) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P * C2) return;
+    const int64_t p = i / C2;
+    const int c = (int)(i - p * C2);
+    gskip[i] = g[p * (C1 + C2) + C1 + c];
+}
+
+// masked mean-square loss over B*H*W pixels of 2 channels:
+// loss = sum m (o - t)^2 / (2 sum m); grad = m (o - t) / sum m
+__global__ void __launch_bounds__(256) k_loss(const float* o, const float* t, const float* mask, int64_t P, float* grad,
+                                              float* out2) {
+    __shared__ float s[2][256];
+    const int tid = threadIdx.x;
+    float se = 0.0f, sm = 0.0f;
+    for (int64_t p = tid; p < P; p += 256) {
+        const float m = mask[p];
+        const float d0 = o[2 * p] - t[2 * p], d1 = o[2 * p + 1] - t[2 * p + 1];
+        se += m * (d0 * d0 + d1 * d1);
+        sm += m;
+    }
+    s[0][tid] = se;
+    s[1][tid] = sm;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+            s[0][tid] += s[0][tid + h];
+            s[1][tid] += s[1][tid + h];
+        }
+        __syncthreads();
+    }
+    const float denom = fmaxf(s[1][0], 1.0f);
+    for (int64_t p = tid; p < P; p += 256) {
+        const float m = mask[p] / denom;
+        grad[2 * p] = m * (o[2 * p] - t[2 * p]);
+        grad[2 * p + 1] = m * (o[2 * p + 1] - t[2 * p + 1]);
+    }
+    if (tid == 0) {
+        out2[0] = s[0][0] / (2.0f * denom);
+        out2[1] = s[1][0];
+    }
+}
+
+__global__ void k_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+template <class Op>
+void gemm(const Op& op, int M, int N, int K, int nz, int nsplit, int kchunk, hipStream_t st) {
+    dim3 grid((M + TM - 1) / TM, (N + TN - 1) / TN, nz);
+    hipLaunchKernelGGL(k_gemm<Op>, grid, dim3(256), 0, st, op, M, N, K, nsplit, kchunk);
+}
+
+}  // namespace net
+
+using namespace net;
+
+// ---------------------------------------------------------------- the network
+struct ConvL {
+    int cin, cout, ksz, relu;
+    int64_t woff, boff;  // offsets in the parameter array
+};
+struct Tens {
+    int C = 0, level = 0;  // level: resolution H >> level
+    bf16* a = nullptr;     // activation
+    float* g = nullptr;    // gradient
+};
+
+struct mts_net_s {
+    hipStream_t stream = nullptr;
+    int c[4];
+    int maxB = 0, maxH = 0, maxW = 0;
+    std::vector<ConvL> L;
+    int64_t nparam = 0;
+    float *P = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;
+    bf16* Pb = nullptr;
+    // tensors
+    enum {
+        X0, E0A, E0, P0, E1A, E1, P1, E2A, E2, P2, E3A, E3, C2, D2A, D2, C1, D1A, D1, C0, D0A, D0, NT
+    };
+    Tens T[NT];
+    float* out = nullptr;   // [B][H][W][2]
+    float* gout = nullptr;  // its gradient
+    bf16* gb = nullptr;     // BF16 gradient operand scratch
+    float* part = nullptr;  // split-K partials
+    int64_t part_n = 0;
+    float* loss = nullptr;  // 2 floats
+    int B = 0, H = 0, W = 0;
+    std::vector<void*> allocs;
+    template <class Tp>
+    int alloc(Tp** p, size_t n) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(Tp));
+        if (e != hipSuccess) return set_error(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = (Tp*)q;
+        return MTS_OK;
+    }
+    ~mts_net_s() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* q : allocs) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int64_t npix(int level) const { return (int64_t)B * (H >> level) * (W >> level); }
+    Geom geom(int level) const { return Geom{B, H >> level, W >> level}; }
+
+    void conv_fwd(int li, const Tens& in, Tens& out_t, float* outf) {
+        const ConvL& l = L[li];
+        OpConvFwd op{geom(in.level), in.a, Pb + l.woff, P + l.boff, out_t.a, outf, l.cin, l.cout, l.ksz, l.relu};
+        gemm(op, (int)npix(in.level), l.cout, l.cin * l.ksz * l.ksz, 1, 1, l.cin * l.ksz * l.ksz, stream);
+    }
+    // gradient of layer li given the FP32 gradient of its output (gy) and its
+    // output activation (y, for the ReLU mask; nullptr for the head)
+    void conv_bwd(int li, const Tens& in, const float* gy, const bf16* y, float* gin, int add) {
+        const ConvL& l = L[li];
+        const int64_t Pn = npix(in.level);
+        const int64_t ny = Pn * l.cout;
+        hipLaunchKernelGGL(k_grad_prep, dim3(nblk(ny)), dim3(256), 0, stream, gy, y, gb, ny);
+        hipLaunchKernelGGL(k_bias_grad, dim3(l.cout), dim3(256), 0, stream, gb, Pn, l.cout, G + l.boff);
+        const int nkk = l.ksz * l.ksz;
+        const int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, (Pn + 8191) / 8192));
+        const int kchunk = (int)(((Pn + nsplit - 1) / nsplit + TK - 1) / TK * TK);
+        OpConvBwdW ow{geom(in.level), gb, in.a, part, l.cin, l.cout, l.ksz, nsplit};
+        gemm(ow, l.cout, l.cin, (int)Pn, nkk * nsplit, nsplit, kchunk, stream);
+        const int64_t per = (int64_t)l.cout * l.cin;
+        hipLaunchKernelGGL(k_sum_parts, dim3(nblk(nkk * per)), dim3(256), 0, stream, part, G + l.woff, nkk, nsplit, per);
+        if (gin) {
+            OpConvBwdData od{geom(in.level), gb, Pb + l.woff, gin, l.cin, l.cout, l.ksz, add};
+            gemm(od, (int)Pn, l.cin, l.cout * nkk, 1, 1, l.cout * nkk, stream);
+        }
+    }
+};
+
+namespace {
+
+int check_net(mts_net n) { return n ? MTS_OK : set_error(MTS_ERR_ARG, "null network handle"); }
+
+// layer table of the U-Net (widths c0..c3, input 3 channels, output 2)
+void layers(mts_net_s* n) {
+    const int* c = n->c;
+    const int spec[15][4] = {{3, c[0], 3, 1},          {c[0], c[0], 3, 1},        {c[0], c[1], 3, 1},
+                             {c[1], c[1], 3, 1},       {c[1], c[2], 3, 1},        {c[2], c[2], 3, 1},
+                             {c[2], c[3], 3, 1},       {c[3], c[3], 3, 1},        {c[3] + c[2], c[2], 3, 1},
+                             {c[2], c[2], 3, 1},       {c[2] + c[1], c[1], 3, 1}, {c[1], c[1], 3, 1},
+                             {c[1] + c[0], c[0], 3, 1}, {c[0], c[0], 3, 1},       {c[0], 2, 1, 0}};
+    int64_t off = 0;
+    for (auto& s : spec) {
+        ConvL l{s[0], s[1], s[2], s[3], 0, 0};
+        l.woff = off;
+        off += (int64_t)l.ksz * l.ksz * l.cout * l.cin;
+        l.boff = off;
+        off += l.cout;
+        n->L.push_back(l);
+    }
+    n->nparam = off;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mts_net_create(int c0, int max_batch, int max_h, int max_w, uint64_t seed, mts_net* out) {
+    if (!out || c0 <= 0 || max_batch <= 0 || max_h <= 0 || max_w <= 0 || max_h % 8 || max_w % 8)
+        return set_error(MTS_ERR_ARG, "bad network shape (sizes must be multiples of 8)");
+    *out = nullptr;
+    mts_net_s* n = new (std::nothrow) mts_net_s();
+    if (!n) return set_error(MTS_ERR_OOM, "host allocation");
+    auto bail = [&](int rc) {
+        delete n;
+        return rc;
+    };
+    n->c[0] = c0;
+    n->c[1] = 2 * c0;
+    n->c[2] = 4 * c0;
+    n->c[3] = 8 * c0;
+    n->maxB = max_batch;
+    n->maxH = max_h;
+    n->maxW = max_w;
+    layers(n);
+    hipError_t e = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(set_error(MTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e)));
+    int rc;
+    const int64_t np = n->nparam;
+    if ((rc = n->alloc(&n->P, np)) || (rc = n->alloc(&n->G, np)) || (rc = n->alloc(&n->Mo, np)) ||
+        (rc = n->alloc(&n->Vo, np)) || (rc = n->alloc(&n->Pb, np)))
+        return bail(rc);
+    // tensors: channels and level
+    const int* c = n->c;
+    const int tc[mts_net_s::NT][2] = {{3, 0},        {c[0], 0}, {c[0], 0},         {c[0], 1}, {c[1], 1},
+                                      {c[1], 1},     {c[1], 2}, {c[2], 2},         {c[2], 2}, {c[2], 3},
+                                      {c[3], 3},     {c[3], 3}, {c[3] + c[2], 2},  {c[2], 2}, {c[2], 2},
+                                      {c[2] + c[1], 1}, {c[1], 1}, {c[1], 1},      {c[1] + c[0], 0}, {c[0], 0},
+                                      {c[0], 0}};
+    int64_t maxe = 0;
+    for (int t = 0; t < mts_net_s::NT; t++) {
+        Tens& T = n->T[t];
+        T.C = tc[t][0];
+        T.level = tc[t][1];
+        const int64_t ne = (int64_t)max_batch * (max_h >> T.level) * (max_w >> T.level) * T.C;
+        maxe = std::max(maxe, ne);
+        if ((rc = n->alloc(&T.a, ne)) || (rc = n->alloc(&T.g, ne))) return bail(rc);
+    }
+    const int64_t P0 = (int64_t)max_batch * max_h * max_w;
+    int64_t pmax = 0;
+    for (const ConvL& l : n->L) {
+        const int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, (P0 + 8191) / 8192));
+        pmax = std::max(pmax, (int64_t)l.ksz * l.ksz * nsplit * l.cout * l.cin);
+    }
+    n->part_n = pmax;
+    if ((rc = n->alloc(&n->out, P0 * 2)) || (rc = n->alloc(&n->gout, P0 * 2)) || (rc = n->alloc(&n->gb, maxe)) ||
+        (rc = n->alloc(&n->part, pmax)) || (rc = n->alloc(&n->loss, 2)))
+        return bail(rc);
+    // He-normal weights, zero biases, zero Adam moments
+    std::vector<float> h(np, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (const ConvL& l : n->L) {
+        std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (float)(l.cin * l.ksz * l.ksz)));
+        const float sc = l.relu ? 1.0f : 0.1f;  // small head: the first predictions stay near zero
+        for (int64_t i = 0; i < (int64_t)l.ksz * l.ksz * l.cout * l.cin; i++) h[l.woff + i] = sc * nd(rng);
+    }
+    if ((e = hipMemcpy(n->P, h.data(), np * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(n->Mo, 0, np * 4)) != hipSuccess || (e = hipMemset(n->Vo, 0, np * 4)) != hipSuccess ||
+        (e = hipMemset(n->G, 0, np * 4)) != hipSuccess)
+        return bail(set_error(MTS_ERR_HIP, std::string("init: ") + hipGetErrorString(e)));
+    hipLaunchKernelGGL(k_f2bf, dim3(nblk(np)), dim3(256), 0, n->stream, n->P, n->Pb, np);
+    if ((e = hipStreamSynchronize(n->stream)) != hipSuccess)
+        return bail(set_error(MTS_ERR_HIP, std::string("init: ") + hipGetErrorString(e)));
+    *out = n;
+    return MTS_OK;
+}
+
+int mts_net_destroy(mts_net n) {
+    delete n;
+    return MTS_OK;
+}
+
+int mts_net_param_count(mts_net n, int64_t* count) {
+    if (int rc = check_net(n)) return rc;
+    if (!count) return set_error(MTS_ERR_ARG, "null argument");
+    *count = n->nparam;
+    return MTS_OK;
+}
+
+int mts_net_get_params(mts_net n, float* host) {
+    if (int rc = check_net(n)) return rc;
+    if (!host) return set_error(MTS_ERR_ARG, "null argument");
+    NET_TRY(hipMemcpyAsync(host, n->P, n->nparam * 4, hipMemcpyDeviceToHost, n->stream));
+    NET_TRY(hipStreamSynchronize(n->stream));
+    return MTS_OK;
+}
+
+int mts_net_set_params(mts_net n, const float* host) {
+    if (int rc = check_net(n)) return rc;
+    if (!host) return set_error(MTS_ERR_ARG, "null argument");
+    NET_TRY(hipMemcpyAsync(n->P, host, n->nparam * 4, hipMemcpyHostToDevice, n->stream));
+    hipLaunchKernelGGL(k_f2bf, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->P, n->Pb, n->nparam);
+    NET_TRY(hipStreamSynchronize(n->stream));
+    return MTS_OK;
+}
+
+int mts_net_get_grads(mts_net n, float* host) {
+    if (int rc = check_net(n)) return rc;
+    if (!host) return set_error(MTS_ERR_ARG, "null argument");
+    NET_TRY(hipMemcpyAsync(host, n->G, n->nparam * 4, hipMemcpyDeviceToHost, n->stream));
+    NET_TRY(hipStreamSynchronize(n->stream));
+    return MTS_OK;
+}
+
+int mts_net_forward(mts_net n, const float* in, int B, int H, int W, float* out) {
+    if (int rc = check_net(n)) return rc;
+    if (!in || B <= 0 || B > n->maxB || H <= 0 || W <= 0 || H > n->maxH || W > n->maxW || H % 8 || W % 8 ||
+        (int64_t)H * W * B > (int64_t)n->maxH * n->maxW * n->maxB)
+        return set_error(MTS_ERR_ARG, "input shape exceeds the network's buffers or is not a multiple of 8");
+    n->B = B;
+    n->H = H;
+    n->W = W;
+    hipStream_t st = n->stream;
+    auto& T = n->T;
+    const int64_t p0 = n->npix(0);
+    hipLaunchKernelGGL(k_f2bf, dim3(nblk(p0 * 3)), dim3(256), 0, st, in, T[mts_net_s::X0].a, p0 * 3);
+    typedef mts_net_s S;
+    auto pool = [&](int a, int b) {
+        const Tens& x = T[a];
+        hipLaunchKernelGGL(k_pool, dim3(nblk(n->npix(x.level + 1) * x.C)), dim3(256), 0, st, x.a, T[b].a, B,
+                           H >> x.level, W >> x.level, x.C);
+    };
+    auto upcat = [&](int lo, int skip, int out_t) {
+        const Tens& s = T[skip];
+        hipLaunchKernelGGL(k_upcat, dim3(nblk(n->npix(s.level) * T[out_t].C)), dim3(256), 0, st, T[lo].a, T[lo].C, s.a,
+                           s.C, T[out_t].a, B, H >> s.level, W >> s.level);
+    };
+    n->conv_fwd(0, T[S::X0], T[S::E0A], nullptr);
+    n->conv_fwd(1, T[S::E0A], T[S::E0], nullptr);
+    pool(S::E0, S::P0);
+    n->conv_fwd(2, T[S::P0], T[S::E1A], nullptr);
+    n->conv_fwd(3, T[S::E1A], T[S::E1], nullptr);
+    pool(S::E1, S::P1);
+    n->conv_fwd(4, T[S::P1], T[S::E2A], nullptr);
+    n->conv_fwd(5, T[S::E2A], T[S::E2], nullptr);
+    pool(S::E2, S::P2);
+    n->conv_fwd(6, T[S::P2], T[S::E3A], nullptr);
+    n->conv_fwd(7, T[S::E3A], T[S::E3], nullptr);
+    upcat(S::E3, S::E2, S::C2);
+    n->conv_fwd(8, T[S::C2], T[S::D2A], nullptr);
+    n->conv_fwd(9, T[S::D2A], T[S::D2], nullptr);
+    upcat(S::D2, S::E1, S::C1);
+    n->conv_fwd(10, T[S::C1], T[S::D1A], nullptr);
+    n->conv_fwd(11, T[S::D1A], T[S::D1], nullptr);
+    upcat(S::D1, S::E0, S::C0);
+    n->conv_fwd(12, T[S::C0], T[S::D0A], nullptr);
+    n->conv_fwd(13, T[S::D0A], T[S::D0], nullptr);
+    Tens dummy;
+    n->conv_fwd(14, T[S::D0], dummy, n->out);
+    if (out) NET_TRY(hipMemcpyAsync(out, n->out, p0 * 2 * 4, hipMemcpyDeviceToDevice, st));
+    NET_TRY(hipGetLastError());
+    NET_TRY(hipStreamSynchronize(st));
+    return MTS_OK;
+}
+
+int mts_net_backward(mts_net n, const float* target, const float* mask, float* loss_host) {
+    if (int rc = check_net(n)) return rc;
+    if (!target || !mask || n->B == 0) return set_error(MTS_ERR_ARG, "run mts_net_forward first");
+    hipStream_t st = n->stream;
+    auto& T = n->T;
+    typedef mts_net_s S;
+    const int B = n->B, H = n->H, W = n->W;
+    const int64_t p0 = n->npix(0);
+    hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, st, n->out, target, mask, p0, n->gout, n->loss);
+    // head, then decoder, then encoder (reverse order of the forward pass)
+    n->conv_bwd(14, T[S::D0], n->gout, nullptr, T[S::D0].g, 0);
+    n->conv_bwd(13, T[S::D0A], T[S::D0].g, T[S::D0].a, T[S::D0A].g, 0);
+    n->conv_bwd(12, T[S::C0], T[S::D0A].g, T[S::D0A].a, T[S::C0].g, 0);
+    auto upcat_bwd = [&](int lo, int skip, int cat) {
+        const Tens& s = T[skip];
+        const int hh = H >> s.level, ww = W >> s.level;
+        hipLaunchKernelGGL(k_upcat_bwd_lo, dim3(nblk(n->npix(s.level + 1) * T[lo].C)), dim3(256), 0, st, T[cat].g,
+                           T[lo].C, s.C, T[lo].g, B, hh, ww);
+        hipLaunchKernelGGL(k_upcat_bwd_skip, dim3(nblk(n->npix(s.level) * s.C)), dim3(256), 0, st, T[cat].g, T[lo].C,
+                           s.C, T[skip].g, n->npix(s.level));
+    };
+    auto pool_bwd = [&](int pooled, int x) {
+        const Tens& t = T[x];
+        hipLaunchKernelGGL(k_pool_bwd, dim3(nblk(n->npix(t.level) * t.C)), dim3(256), 0, st, T[pooled].g, t.g, B,
+                           H >> t.level, W >> t.level, t.C);
+    };
+    upcat_bwd(S::D1, S::E0, S::C0);  // D1.g, E0.g (first part)
+    n->conv_bwd(11, T[S::D1A], T[S::D1].g, T[S::D1].a, T[S::D1A].g, 0);
+    n->conv_bwd(10, T[S::C1], T[S::D1A].g, T[S::D1A].a, T[S::C1].g, 0);
+    upcat_bwd(S::D2, S::E1, S::C1);
+    n->conv_bwd(9, T[S::D2A], T[S::D2].g, T[S::D2].a, T[S::D2A].g, 0);
+    n->conv_bwd(8, T[S::C2], T[S::D2A].g, T[S::D2A].a, T[S::C2].g, 0);
+    upcat_bwd(S::E3, S::E2, S::C2);
+    n->conv_bwd(7, T[S::E3A], T[S::E3].g, T[S::E3].a, T[S::E3A].g, 0);
+    n->conv_bwd(6, T[S::P2], T[S::E3A].g, T[S::E3A].a, T[S::P2].g, 0);
+    pool_bwd(S::P2, S::E2);  // adds into E2.g
+    n->conv_bwd(5, T[S::E2A], T[S::E2].g, T[S::E2].a, T[S::E2A].g, 0);
+    n->conv_bwd(4, T[S::P1], T[S::E2A].g, T[S::E2A].a, T[S::P1].g, 0);
+    pool_bwd(S::P1, S::E1);
+    n->conv_bwd(3, T[S::E1A], T[S::E1].g, T[S::E1].a, T[S::E1A].g, 0);
+    n->conv_bwd(2, T[S::P0], T[S::E1A].g, T[S::E1A].a, T[S::P0].g, 0);
+    pool_bwd(S::P0, S::E0);
+    n->conv_bwd(1, T[S::E0A], T[S::E0].g, T[S::E0].a, T[S::E0A].g, 0);
+    n->conv_bwd(0, T[S::X0], T[S::E0A].g, T[S::E0A].a, nullptr, 0);
+    NET_TRY(hipGetLastError());
+    float l2[2];
+    NET_TRY(hipMemcpyAsync(l2, n->loss, 8, hipMemcpyDeviceToHost, st));
+    NET_TRY(hipStreamSynchronize(st));
+    if (loss_host) *loss_host = l2[0];
+    return MTS_OK;
+}
+
+int mts_net_adam(mts_net n, float lr, float beta1, float beta2, float eps, int step) {
+    if (int rc = check_net(n)) return rc;
+    if (step < 1) return set_error(MTS_ERR_ARG, "Adam step counts from 1");
+    const float c1 = 1.0f - std::pow(beta1, (float)step), c2 = 1.0f - std::pow(beta2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->P, n->G, n->Mo, n->Vo, n->nparam, lr,
+                       beta1, beta2, eps, c1, c2);
+    hipLaunchKernelGGL(k_f2bf, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->P, n->Pb, n->nparam);
+    NET_TRY(hipGetLastError());
+    NET_TRY(hipStreamSynchronize(n->stream));
+    return MTS_OK;
+}
+
+// The GEMM kernel on dense host matrices (A: M x K, B: K x N, row-major,
+// rounded to BF16 on the device): C = A B in FP32.
+int mts_net_gemm_check(const float* A, const float* Bm, int M, int N, int K, float* C) {
+    if (!A || !Bm || !C || M <= 0 || N <= 0 || K <= 0 || (int64_t)M * N > (1 << 24) || (int64_t)M * K > (1 << 24) ||
+        (int64_t)K * N > (1 << 24))
+        return set_error(MTS_ERR_ARG, "bad GEMM check shape");
+    float *dA, *dB, *dC;
+    bf16 *hA, *hB;
+    NET_TRY(hipMalloc(&dA, (size_t)M * K * 4));
+    NET_TRY(hipMalloc(&dB, (size_t)K * N * 4));
+    NET_TRY(hipMalloc(&dC, (size_t)M * N * 4));
+    NET_TRY(hipMalloc(&hA, (size_t)M * K * 2));
+    NET_TRY(hipMalloc(&hB, (size_t)K * N * 2));
+    NET_TRY(hipMemcpy(dA, A, (size_t)M * K * 4, hipMemcpyHostToDevice));
+    NET_TRY(hipMemcpy(dB, Bm, (size_t)K * N * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_f2bf, dim3(nblk((int64_t)M * K)), dim3(256), 0, nullptr, dA, hA, (int64_t)M * K);
+    hipLaunchKernelGGL(k_f2bf, dim3(nblk((int64_t)K * N)), dim3(256), 0, nullptr, dB, hB, (int64_t)K * N);
+    OpDense op{hA, hB, dC, K, N};
+    gemm(op, M, N, K, 1, 1, K, nullptr);
+    NET_TRY(hipGetLastError());
+    NET_TRY(hipMemcpy(C, dC, (size_t)M * N * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    (void)hipFree(dC);
+    (void)hipFree(hA);
+    (void)hipFree(hB);
+    return MTS_OK;
+}
+
+int mts_net_stream(mts_net n, void** stream) {
+    if (int rc = check_net(n)) return rc;
+    if (!stream) return set_error(MTS_ERR_ARG, "null argument");
+    *stream = (void*)n->stream;
+    return MTS_OK;
+}
+
+}  // extern "C"

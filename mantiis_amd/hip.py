@@ -34,7 +34,10 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows",
            "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy", "mts_batch_create", "mts_batch_destroy",
            "mts_batch_cells", "mts_batch_run", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
-           "mts_batch_stream"]
+           "mts_batch_stream", "mts_dev_alloc", "mts_dev_free", "mts_dev_upload", "mts_dev_download",
+           "mts_net_create", "mts_net_destroy", "mts_net_param_count", "mts_net_get_params", "mts_net_set_params",
+           "mts_net_get_grads", "mts_net_forward", "mts_net_backward", "mts_net_adam", "mts_net_gemm_check",
+           "mts_net_stream"]
 
 
 class HipLibraryMissing(RuntimeError):
@@ -111,6 +114,10 @@ def load_library(path=LIB_PATH):
     lib.mts_batch_fields.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
     lib.mts_batch_set_state.argtypes = [C.c_void_p, C.c_int, _D]
     lib.mts_batch_stream.argtypes = [C.c_void_p, _PP]
+    lib.mts_dev_alloc.argtypes = [C.c_size_t, _PP]
+    lib.mts_dev_free.argtypes = [C.c_void_p]
+    lib.mts_dev_upload.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    lib.mts_dev_download.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     _LIB = lib
     return lib
 
@@ -399,3 +406,128 @@ class Batch:
         s = C.c_void_p()
         _check(self.lib, self.lib.mts_batch_stream(self.h, C.byref(s)))
         return s.value
+
+
+class DeviceArray:
+    """A device buffer of the library (mts_dev_*) with numpy up/download."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.lib = load_library()
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(self.lib, self.lib.mts_dev_alloc(max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes
+        _check(self.lib, self.lib.mts_dev_upload(self.ptr, a.ctypes.data, self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(self.lib, self.lib.mts_dev_download(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def data_ptr(self):
+        return self.ptr
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.lib.mts_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Net:
+    """The warm-start encoder-decoder (mts_net_*).  Inputs, targets, masks
+    and outputs are DeviceArray buffers; parameters are host numpy arrays."""
+
+    def __init__(self, c0=24, max_batch=8, max_h=256, max_w=256, seed=1):
+        self.lib = load_library()
+        for name, args in (("mts_net_create", [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+                           ("mts_net_destroy", [C.c_void_p]), ("mts_net_param_count", [C.c_void_p, _I64]),
+                           ("mts_net_get_params", [C.c_void_p, C.POINTER(C.c_float)]),
+                           ("mts_net_set_params", [C.c_void_p, C.POINTER(C.c_float)]),
+                           ("mts_net_get_grads", [C.c_void_p, C.POINTER(C.c_float)]),
+                           ("mts_net_forward", [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+                           ("mts_net_backward", [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+                           ("mts_net_adam", [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]),
+                           ("mts_net_stream", [C.c_void_p, C.POINTER(C.c_void_p)])):
+            getattr(self.lib, name).argtypes = args
+        h = C.c_void_p()
+        _check(self.lib, self.lib.mts_net_create(c0, max_batch, max_h, max_w, seed, C.byref(h)))
+        self.h = h
+        self.c0 = c0
+        n = C.c_int64()
+        _check(self.lib, self.lib.mts_net_param_count(self.h, C.byref(n)))
+        self.nparam = n.value
+        self.t = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mts_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.nparam, np.float32)
+        _check(self.lib, self.lib.mts_net_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(self.lib, self.lib.mts_net_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def grads(self):
+        out = np.zeros(self.nparam, np.float32)
+        _check(self.lib, self.lib.mts_net_get_grads(self.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def forward(self, x, out=None):
+        """x: DeviceArray float32 [B, H, W, 3]; out: [B, H, W, 2] or None."""
+        B, H, W, _ = x.shape
+        _check(self.lib, self.lib.mts_net_forward(self.h, x.data_ptr(), B, H, W,
+                                                  out.data_ptr() if out is not None else None))
+        return out
+
+    def backward(self, target, mask):
+        loss = C.c_float()
+        _check(self.lib, self.lib.mts_net_backward(self.h, target.data_ptr(), mask.data_ptr(), C.byref(loss)))
+        return loss.value
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.t += 1
+        _check(self.lib, self.lib.mts_net_adam(self.h, lr, beta1, beta2, eps, self.t))
+
+
+def gemm_check(A, B):
+    """C = A B through the network's matrix-core GEMM kernel (BF16 operands)."""
+    lib = load_library()
+    f = C.POINTER(C.c_float)
+    lib.mts_net_gemm_check.argtypes = [f, f, C.c_int, C.c_int, C.c_int, f]
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    M, K = A.shape
+    N = B.shape[1]
+    Cm = np.zeros((M, N), np.float32)
+    _check(lib, lib.mts_net_gemm_check(A.ctypes.data_as(f), B.ctypes.data_as(f), M, N, K, Cm.ctypes.data_as(f)))
+    return Cm

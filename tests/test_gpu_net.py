@@ -1,0 +1,93 @@
+"""The warm-start network's kernels: the matrix-core GEMM against numpy on
+BF16-rounded operands, and a forward/backward pass against the CPU oracle
+(oracle/unet_oracle.py, which rounds to BF16 where the device does)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from _gpu_util import banner
+
+pytestmark = pytest.mark.gpu
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import unet_oracle as U  # noqa: E402
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (100, 37, 70), (16, 130, 300), (257, 65, 9)])
+def test_gemm_matches_numpy(M, N, K):
+    from mantiis_amd.hip import gemm_check
+    banner(f"matrix-core GEMM {M}x{N}x{K}")
+    rng = np.random.default_rng(M * 7 + N)
+    A = U.bf16(rng.normal(size=(M, K)).astype(np.float32))
+    B = U.bf16(rng.normal(size=(K, N)).astype(np.float32))
+    Cg = gemm_check(A, B)
+    Cr = A.astype(np.float64) @ B.astype(np.float64)
+    err = np.abs(Cg - Cr).max() / np.abs(Cr).max()
+    print("max rel err", err)
+    assert err < 1e-6
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_forward_backward_match_oracle():
+    from mantiis_amd.hip import DeviceArray, Net
+    banner("network c0=8, 2x32x32: forward, loss and every gradient vs the CPU oracle")
+    c0, B, H, W = 8, 2, 32, 32
+    net = Net(c0=c0, max_batch=B, max_h=H, max_w=W, seed=3)
+    assert net.nparam == U.param_count(c0)
+    rng = np.random.default_rng(5)
+    x = rng.normal(0, 1, (B, H, W, 3)).astype(np.float32)
+    t = rng.normal(0, 0.5, (B, H, W, 2)).astype(np.float32)
+    m = (rng.uniform(size=(B, H, W)) > 0.4).astype(np.float32)
+    p = net.params()
+    xd, td, md = (DeviceArray.from_numpy(a) for a in (x, t, m))
+    out = DeviceArray((B, H, W, 2))
+    net.forward(xd, out)
+    loss = net.backward(td, md)
+    g = net.grads()
+    o_out, o_loss, o_g = U.forward_backward(p, c0, x, t, m, use_bf16=True)
+    e_out = rel(out.numpy(), o_out)
+    print("out rel", e_out, "loss", loss, o_loss)
+    assert e_out < 2e-2 and abs(loss - o_loss) <= 2e-2 * abs(o_loss)
+    off = 0
+    worst = 0.0
+    for i, (cin, cout, k, relu) in enumerate(U.layer_table(c0)):
+        n = k * k * cout * cin + cout
+        e = rel(g[off:off + n], o_g[off:off + n])
+        worst = max(worst, e)
+        print("layer", i, "grad rel", e)
+        off += n
+    assert worst < 5e-2
+
+
+def test_training_reduces_the_loss():
+    """60 Adam steps on a learnable batch (the target is a local average of
+    the mask scaled by the second input channel): the float64 oracle goes
+    from 0.60 to 0.042 on the same problem."""
+    from mantiis_amd.hip import DeviceArray, Net
+    banner("network c0=8: 60 Adam steps on one batch")
+    B, H, W = 4, 32, 32
+    net = Net(c0=8, max_batch=B, max_h=H, max_w=W, seed=1)
+    rng = np.random.default_rng(2)
+    x = np.zeros((B, H, W, 3), np.float32)
+    t = np.zeros((B, H, W, 2), np.float32)
+    for b in range(B):
+        mk = (rng.uniform(size=(H, W)) > 0.35).astype(np.float32)
+        x[b, ..., 0] = mk
+        x[b, ..., 1] = 0.5 * b
+        x[b, ..., 2] = 1.0
+        box = sum(np.roll(np.roll(mk, dy, 0), dx, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)) / 9
+        t[b, ..., 0] = mk * box * (1 + 0.5 * b)
+        t[b, ..., 1] = -0.5 * mk * box
+    m = x[..., 0].copy()
+    xd, td, md = (DeviceArray.from_numpy(a) for a in (x, t, m))
+    losses = []
+    for _ in range(60):
+        net.forward(xd)
+        losses.append(net.backward(td, md))
+        net.adam(lr=2e-3)
+    print("loss", losses[0], "->", losses[-1])
+    assert losses[-1] < 0.2 * losses[0]
