@@ -1,0 +1,44 @@
+"""What the learned warm start gains: a small network trained on steady
+states of the batched solver seeds unseen lattices, which then reach the
+same tolerance in fewer steps than from rest.  Both runs stop at the same
+per-step tolerance; their permeabilities are compared with a tightly
+converged reference (a warm start approaches from either side, so its
+stopping error can differ in sign from the cold start's)."""
+import numpy as np
+import pytest
+
+from _gpu_util import banner
+
+pytestmark = pytest.mark.gpu
+F = 1e-6
+
+
+def test_warm_start_saves_steps_on_unseen_lattices():
+    from mantiis_amd import warmstart as WS
+    from mantiis_amd.hip import globals_for
+    banner("warm start: train on 384 steady 128^2 lattices, seed 16 unseen ones")
+    rho_in = globals_for(dx=1e-9)[1]["rhoin"]
+    train_s = WS.samples(range(100, 484), 128, workers=8)
+    r, fields = WS.solve(train_s, F)
+    assert (r["steps"] > 0).all()
+    ds = WS.Dataset([WS.features(s, F) for s in train_s], [WS.targets(s, f, F) for s, f in zip(train_s, fields)])
+    net, hist = WS.train(ds, c0=8, batch=8, epochs=30, lr=2e-3)
+    print("training loss", hist[0], "->", hist[-1])
+    assert hist[-1] < 0.5 * hist[0]
+    test_s = WS.samples(range(900, 916), 128, workers=8)
+    rc, fc = WS.solve(test_s, F)
+    pred = WS.predict(net.params(), 8, test_s, F)
+    warm = [WS.seed_populations(s, ux, uy, rho_in) for s, (ux, uy) in zip(test_s, pred)]
+    rw, fw = WS.solve(test_s, F, warm=warm)
+    kc = np.array([WS.permeability(s, f[1], F) for s, f in zip(test_s, fc)])
+    kw = np.array([WS.permeability(s, f[1], F) for s, f in zip(test_s, fw)])
+    kp = np.array([WS.permeability(s, p[0], F) for s, p in zip(test_s, pred)])
+    rr, fr = WS.solve(test_s, F, tol=1e-9, max_steps=300000)
+    kr = np.array([WS.permeability(s, f[1], F) for s, f in zip(test_s, fr)])
+    print("cold steps", rc["steps"], "warm steps", rw["steps"])
+    print("k error per lattice: network", np.round(kp / kr - 1, 3))
+    print("k error vs the 1e-9 reference: network", np.abs(kp / kr - 1).max(), "cold", np.abs(kc / kr - 1).max(),
+          "warm", np.abs(kw / kr - 1).max())
+    assert (rc["steps"] > 0).all() and (rw["steps"] > 0).all()
+    assert rw["steps"].sum() < rc["steps"].sum()
+    assert np.abs(kc / kr - 1).max() < 0.05 and np.abs(kw / kr - 1).max() < 0.05
