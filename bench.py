@@ -273,12 +273,92 @@ def bench_c4(args):
     return 0
 
 
+def bench_c5(args):
+    """Learned warm start: train the encoder-decoder on steady states of
+    --train 256^2 lattices of the batched distribution (from the batched
+    solver), then solve --eval unseen 512^2 lattices from rest and from the
+    network's prediction to the same tolerance; value = plain wall time /
+    warm-started wall time of the batched solve."""
+    from mantiis_amd import warmstart as WS
+    from mantiis_amd.hip import globals_for
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import unet_oracle as U
+    F = 1e-6
+    rho_in = globals_for(dx=1e-9)[1]["rhoin"]
+    t0 = time.perf_counter()
+    train_s = WS.samples(range(10000, 10000 + args.train), 256)
+    r, fields = WS.solve(train_s, F)
+    t_data = time.perf_counter() - t0
+    ds = WS.Dataset([WS.features(s, F) for s in train_s], [WS.targets(s, f, F) for s, f in zip(train_s, fields)])
+    del fields
+    c0, B = 24, 8
+    t0 = time.perf_counter()
+    net, hist = WS.train(ds, c0=c0, batch=B, epochs=args.epochs, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    steps_train = args.epochs * (ds.n // B)
+    flops = 0
+    lv = [0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0]
+    for i, (cin, cout, k, relu) in enumerate(U.layer_table(c0)):
+        flops += 2 * B * (256 >> lv[i]) ** 2 * cin * cout * k * k
+    tflops = 3 * flops * steps_train / t_train / 1e12  # forward + two backward products per layer
+    params = net.params()
+    net.close()
+    ds = None
+    eval_s = WS.samples(range(20000, 20000 + args.eval), 512)
+    t0 = time.perf_counter()
+    rc, fc = WS.solve(eval_s, F)
+    t_cold = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pred = WS.predict(params, c0, eval_s, F)
+    t_pred = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    warm = [WS.seed_populations(s, ux, uy, rho_in) for s, (ux, uy) in zip(eval_s, pred)]
+    t_seed = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rw, fw = WS.solve(eval_s, F, warm=warm)
+    t_warm = time.perf_counter() - t0
+    kc = np.array([WS.permeability(s, f[1], F) for s, f in zip(eval_s, fc)])
+    kw = np.array([WS.permeability(s, f[1], F) for s, f in zip(eval_s, fw)])
+    kp = np.array([WS.permeability(s, p[0], F) for s, p in zip(eval_s, pred)])
+    warm_total = t_pred + t_seed + t_warm
+    out = {
+        "metric": "MLUPS & %HBM roofline; wall-time to steady permeability (warm-start vs plain)",
+        "value": round(t_cold / warm_total, 3),
+        "unit": "x (plain / warm-started wall time to steady state, batched)",
+        "n_gpus": 1, "steps": int(rw["steps"].sum()), "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (network, fp32 accumulation) / fp32 (LBM)",
+        "data": "synthetic (seeded disk packings); training targets from the batched solver",
+        "config": {"workload": f"learned warm start: train {args.train} x 256^2 lattices of the batched distribution "
+                               f"({args.epochs} epochs, U-Net c0={c0}, {len(params)} parameters), infer {args.eval} "
+                               "unseen 512^2 lattices, then LBM to |d<u_x>/dt|/|<u_x>| < 1e-6 per step",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "kernel": "k_gemm (BF16 MFMA implicit GEMM, training step)",
+                     "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 5),
+                     "traffic": None},
+        "cpu_baseline": None,
+        "warm_start": {"plain_s": round(t_cold, 3), "warm_s": round(warm_total, 3), "predict_s": round(t_pred, 3),
+                       "seed_s": round(t_seed, 3), "warm_lbm_s": round(t_warm, 3),
+                       "plain_steps": int(rc["steps"].sum()), "warm_steps": int(rw["steps"].sum()),
+                       "k_error_network": {"median": float(np.median(np.abs(kp / kc - 1))),
+                                           "max": float(np.abs(kp / kc - 1).max())},
+                       "k_error_warm_vs_plain": {"median": float(np.median(np.abs(kw / kc - 1))),
+                                                 "max": float(np.abs(kw / kc - 1).max())},
+                       "training": {"data_s": round(t_data, 1), "train_s": round(t_train, 1),
+                                    "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1]}},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "transport", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "transport", "c1"])
+    ap.add_argument("--train", type=int, default=2000, help="c5: training lattices (256^2)")
+    ap.add_argument("--eval", type=int, default=64, help="c5: unseen evaluation lattices (512^2)")
+    ap.add_argument("--epochs", type=int, default=20, help="c5: training epochs")
     ap.add_argument("--batch", type=int, default=256, help="c4: number of lattices")
     ap.add_argument("--no-steady", action="store_true", help="c2: skip the run to steady state")
     ap.add_argument("--no-cpu", action="store_true")
@@ -315,6 +395,8 @@ def main():
         return 0
     if args.workload == "c4":
         return bench_c4(args)
+    if args.workload == "c5":
+        return bench_c5(args)
     if args.workload == "c3":
         lat, inputs = make_c3()
         kind = "adsorption"

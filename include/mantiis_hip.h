@@ -278,7 +278,7 @@ int mts_batch_stream(mts_batch b, void** stream);
  * channels (scaled velocity), trained with its own BF16 matrix-core forward
  * and backward kernels.  Inputs, targets, masks and outputs are device
  * pointers, NHWC FP32: in [B][H][W][3], out/target [B][H][W][2], mask
- * [B][H][W]; H and W multiples of 8 within the sizes given at creation.
+ * [B][H][W]; c0, H and W multiples of 8, within the sizes given at creation.
  * ---------------------------------------------------------------------- */
 typedef struct mts_net_s* mts_net;
 int mts_net_create(int c0, int max_batch, int max_h, int max_w, uint64_t seed, mts_net* out);

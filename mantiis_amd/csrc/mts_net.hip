@@ -66,15 +66,9 @@ __global__ void __launch_bounds__(256) k_gemm(Op op, int M, int N, int K, int ns
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int r = t >> 2, kq = (t & 3) * 8;
     for (int k0 = kb; k0 < ke; k0 += TK) {
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int k = k0 + kq + e;
-            const bool kin = k < ke;
-            sA[r][kq + e] = (kin && m0 + r < M) ? op.a(m0 + r, k, z) : (bf16)0.0f;
-            sB[r][kq + e] = (kin && n0 + r < N) ? op.b(k, n0 + r, z) : (bf16)0.0f;
-        }
+        op.fill_a(sA, m0, k0, ke, M, z, t);
+        op.fill_b(sB, n0, k0, ke, N, z, t);
         __syncthreads();
         const int i = l & 15, q = (l >> 4) * 8;
         bf16x8 af[2], bfm[2];
@@ -100,13 +94,50 @@ __global__ void __launch_bounds__(256) k_gemm(Op op, int M, int N, int K, int ns
             }
 }
 
+// LDS tile fills.  Row fill: thread t takes row t/4 and the 8 consecutive k
+// from 8 (t%4); one 16-byte load when the operand keeps those 8 k
+// contiguous (pa/pb), element loads otherwise.  Transposed fill (operands
+// whose 8 contiguous elements run along the tile's row index, e.g. 8
+// channels of one pixel): thread t takes rows 8 (t%8) .. +7 at k = t/8.
+typedef uint4 v16;
+
+template <class Op>
+__device__ __forceinline__ void fill_rows_a(const Op& op, bf16 (*s)[LDK], int m0, int k0, int ke, int M, int z, int t) {
+    const int r = t >> 2, kq = (t & 3) * 8, m = m0 + r, k = k0 + kq;
+    if (op.vec_a && m < M && k + 7 < ke) {
+        *(v16*)&s[r][kq] = *(const v16*)op.pa(m, k, z);
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) s[r][kq + e] = (m < M && k + e < ke) ? op.a(m, k + e, z) : (bf16)0.0f;
+}
+template <class Op>
+__device__ __forceinline__ void fill_rows_b(const Op& op, bf16 (*s)[LDK], int n0, int k0, int ke, int N, int z, int t) {
+    const int r = t >> 2, kq = (t & 3) * 8, n = n0 + r, k = k0 + kq;
+    if (op.vec_b && n < N && k + 7 < ke) {
+        *(v16*)&s[r][kq] = *(const v16*)op.pb(k, n, z);
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) s[r][kq + e] = (n < N && k + e < ke) ? op.b(k + e, n, z) : (bf16)0.0f;
+}
+
 // dense row-major operands (the GEMM check)
 struct OpDense {
     const bf16 *A, *B;
     float* C;
     int K, N;
+    static constexpr bool vec_a = false, vec_b = false;
     __device__ bf16 a(int m, int k, int) const { return A[(int64_t)m * K + k]; }
     __device__ bf16 b(int k, int n, int) const { return B[(int64_t)k * N + n]; }
+    __device__ const bf16* pa(int, int, int) const { return nullptr; }
+    __device__ const bf16* pb(int, int, int) const { return nullptr; }
+    __device__ void fill_a(bf16 (*s)[LDK], int m0, int k0, int ke, int M, int z, int t) const {
+        fill_rows_a(*this, s, m0, k0, ke, M, z, t);
+    }
+    __device__ void fill_b(bf16 (*s)[LDK], int n0, int k0, int ke, int N, int z, int t) const {
+        fill_rows_b(*this, s, n0, k0, ke, N, z, t);
+    }
     __device__ void store(int m, int n, float v, int) const { C[(int64_t)m * N + n] = v; }
 };
 
@@ -134,17 +165,26 @@ struct OpConvFwd {
     bf16* Y;         // [p][Cout] (BF16, ReLU) or
     float* Yf;       // [p][Cout] (FP32, no ReLU)
     int Cin, Cout, ksz, relu;
-    __device__ bf16 a(int m, int k, int) const {
+    bool vec_a, vec_b;  // Cin % 8 == 0: 8 consecutive k are 8 channels of one kk
+    __device__ const bf16* pa(int m, int k, int) const {
         const int kk = k / Cin, ci = k - kk * Cin;
         int b, y, x, dy, dx;
         g.pix(m, b, y, x);
         koff(kk, ksz, dy, dx);
         const int yy = wrapi(y + dy, g.H), xx = wrapi(x + dx, g.W);
-        return X[(((int64_t)b * g.H + yy) * g.W + xx) * Cin + ci];
+        return X + (((int64_t)b * g.H + yy) * g.W + xx) * Cin + ci;
     }
-    __device__ bf16 b(int k, int n, int) const {
+    __device__ const bf16* pb(int k, int n, int) const {
         const int kk = k / Cin, ci = k - kk * Cin;
-        return Wt[((int64_t)kk * Cout + n) * Cin + ci];
+        return Wt + ((int64_t)kk * Cout + n) * Cin + ci;
+    }
+    __device__ bf16 a(int m, int k, int z) const { return *pa(m, k, z); }
+    __device__ bf16 b(int k, int n, int z) const { return *pb(k, n, z); }
+    __device__ void fill_a(bf16 (*s)[LDK], int m0, int k0, int ke, int M, int z, int t) const {
+        fill_rows_a(*this, s, m0, k0, ke, M, z, t);
+    }
+    __device__ void fill_b(bf16 (*s)[LDK], int n0, int k0, int ke, int N, int z, int t) const {
+        fill_rows_b(*this, s, n0, k0, ke, N, z, t);
     }
     __device__ void store(int m, int n, float v, int) const {
         v += bias[n];
@@ -156,21 +196,30 @@ struct OpConvFwd {
 
 struct OpConvBwdData {
     Geom g;
-    const bf16* dY;  // [p][Cout]
-    const bf16* Wt;  // [kk][Cout][Cin]
-    float* dX;       // [p][Cin]
+    const bf16* dY;   // [p][Cout]
+    const bf16* WtT;  // [kk][Cin][Cout] (transposed copy)
+    float* dX;        // [p][Cin]
     int Cin, Cout, ksz, add;
-    __device__ bf16 a(int m, int k, int) const {
+    bool vec_a, vec_b;  // Cout % 8 == 0
+    __device__ const bf16* pa(int m, int k, int) const {
         const int kk = k / Cout, co = k - kk * Cout;
         int b, y, x, dy, dx;
         g.pix(m, b, y, x);
         koff(kk, ksz, dy, dx);
         const int yy = wrapi(y - dy, g.H), xx = wrapi(x - dx, g.W);
-        return dY[(((int64_t)b * g.H + yy) * g.W + xx) * Cout + co];
+        return dY + (((int64_t)b * g.H + yy) * g.W + xx) * Cout + co;
     }
-    __device__ bf16 b(int k, int n, int) const {
+    __device__ const bf16* pb(int k, int n, int) const {
         const int kk = k / Cout, co = k - kk * Cout;
-        return Wt[((int64_t)kk * Cout + co) * Cin + n];
+        return WtT + ((int64_t)kk * Cin + n) * Cout + co;
+    }
+    __device__ bf16 a(int m, int k, int z) const { return *pa(m, k, z); }
+    __device__ bf16 b(int k, int n, int z) const { return *pb(k, n, z); }
+    __device__ void fill_a(bf16 (*s)[LDK], int m0, int k0, int ke, int M, int z, int t) const {
+        fill_rows_a(*this, s, m0, k0, ke, M, z, t);
+    }
+    __device__ void fill_b(bf16 (*s)[LDK], int n0, int k0, int ke, int N, int z, int t) const {
+        fill_rows_b(*this, s, n0, k0, ke, N, z, t);
     }
     __device__ void store(int m, int n, float v, int) const {
         float* o = dX + (int64_t)m * Cin + n;
@@ -184,14 +233,47 @@ struct OpConvBwdW {
     const bf16* X;   // [p][Cin]
     float* part;     // [z][Cout][Cin], z = kk * nsplit + s
     int Cin, Cout, ksz, nsplit;
-    __device__ bf16 a(int m, int k, int) const { return dY[(int64_t)k * Cout + m]; }
-    __device__ bf16 b(int k, int n, int z) const {
+    bool vec_a, vec_b;  // Cout % 8 == 0, Cin % 8 == 0
+    __device__ int64_t xrow(int k, int z) const {  // site of pixel k shifted by the offset of z's kk
         const int kk = z / nsplit;
         int b, y, x, dy, dx;
         g.pix(k, b, y, x);
         koff(kk, ksz, dy, dx);
         const int yy = wrapi(y + dy, g.H), xx = wrapi(x + dx, g.W);
-        return X[(((int64_t)b * g.H + yy) * g.W + xx) * Cin + n];
+        return ((int64_t)b * g.H + yy) * g.W + xx;
+    }
+    __device__ bf16 a(int m, int k, int) const { return dY[(int64_t)k * Cout + m]; }
+    __device__ bf16 b(int k, int n, int z) const { return X[xrow(k, z) * Cin + n]; }
+    // transposed fills: thread t loads the 8 channels 8 (t%8).. of pixel k0 + t/8
+    __device__ void fill_a(bf16 (*s)[LDK], int m0, int k0, int ke, int M, int, int t) const {
+        const int gq = (t & 7) * 8, p = t >> 3, m = m0 + gq, k = k0 + p;
+        if (vec_a && m + 7 < M && k < ke) {
+            const v16 v = *(const v16*)(dY + (int64_t)k * Cout + m);
+            const bf16* e8 = (const bf16*)&v;
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[gq + e][p] = e8[e];
+            return;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[gq + e][p] = (m + e < M && k < ke) ? dY[(int64_t)k * Cout + m + e] : (bf16)0.0f;
+    }
+    __device__ void fill_b(bf16 (*s)[LDK], int n0, int k0, int ke, int N, int z, int t) const {
+        const int gq = (t & 7) * 8, p = t >> 3, n = n0 + gq, k = k0 + p;
+        if (k >= ke) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[gq + e][p] = (bf16)0.0f;
+            return;
+        }
+        const bf16* row = X + xrow(k, z) * Cin;
+        if (vec_b && n + 7 < N) {
+            const v16 v = *(const v16*)(row + n);
+            const bf16* e8 = (const bf16*)&v;
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[gq + e][p] = e8[e];
+            return;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[gq + e][p] = n + e < N ? row[n + e] : (bf16)0.0f;
     }
     __device__ void store(int m, int n, float v, int z) const { part[((int64_t)z * Cout + m) * Cin + n] = v; }
 };
@@ -345,6 +427,17 @@ __global__ void k_adam(float* w, const float* g, float* m, float* v, int64_t n, 
     w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// BF16 copy of one layer's weights transposed to [kk][cin][cout]
+__global__ void k_wtrans(const float* w, bf16* wt, int nkk, int cout, int cin) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)cout * cin;
+    if (i >= nkk * per) return;
+    const int kk = (int)(i / per);
+    const int64_t e = i - kk * per;
+    const int co = (int)(e / cin), ci = (int)(e - (int64_t)co * cin);
+    wt[(int64_t)kk * per + (int64_t)ci * cout + co] = (bf16)w[i];
+}
+
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 template <class Op>
@@ -375,7 +468,8 @@ struct mts_net_s {
     std::vector<ConvL> L;
     int64_t nparam = 0;
     float *P = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;
-    bf16* Pb = nullptr;
+    bf16* Pb = nullptr;   // BF16 copy, weights [kk][cout][cin]
+    bf16* PbT = nullptr;  // BF16 copy, weights [kk][cin][cout]
     // tensors
     enum {
         X0, E0A, E0, P0, E1A, E1, P1, E2A, E2, P2, E3A, E3, C2, D2A, D2, C1, D1A, D1, C0, D0A, D0, NT
@@ -404,11 +498,22 @@ struct mts_net_s {
         if (stream) (void)hipStreamDestroy(stream);
     }
     int64_t npix(int level) const { return (int64_t)B * (H >> level) * (W >> level); }
+    // BF16 copies of the FP32 parameters (both weight layouts)
+    void refresh_bf16() {
+        hipLaunchKernelGGL(k_f2bf, dim3(nblk(nparam)), dim3(256), 0, stream, P, Pb, nparam);
+        for (const ConvL& l : L) {
+            const int nkk = l.ksz * l.ksz;
+            const int64_t nw = (int64_t)nkk * l.cout * l.cin;
+            hipLaunchKernelGGL(k_wtrans, dim3(nblk(nw)), dim3(256), 0, stream, P + l.woff, PbT + l.woff, nkk, l.cout,
+                               l.cin);
+        }
+    }
     Geom geom(int level) const { return Geom{B, H >> level, W >> level}; }
 
     void conv_fwd(int li, const Tens& in, Tens& out_t, float* outf) {
         const ConvL& l = L[li];
-        OpConvFwd op{geom(in.level), in.a, Pb + l.woff, P + l.boff, out_t.a, outf, l.cin, l.cout, l.ksz, l.relu};
+        OpConvFwd op{geom(in.level), in.a, Pb + l.woff, P + l.boff, out_t.a,  outf,
+                     l.cin,          l.cout, l.ksz,       l.relu,      l.cin % 8 == 0, l.cin % 8 == 0};
         gemm(op, (int)npix(in.level), l.cout, l.cin * l.ksz * l.ksz, 1, 1, l.cin * l.ksz * l.ksz, stream);
     }
     // gradient of layer li given the FP32 gradient of its output (gy) and its
@@ -422,12 +527,13 @@ struct mts_net_s {
         const int nkk = l.ksz * l.ksz;
         const int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, (Pn + 8191) / 8192));
         const int kchunk = (int)(((Pn + nsplit - 1) / nsplit + TK - 1) / TK * TK);
-        OpConvBwdW ow{geom(in.level), gb, in.a, part, l.cin, l.cout, l.ksz, nsplit};
+        OpConvBwdW ow{geom(in.level), gb, in.a, part, l.cin, l.cout, l.ksz, nsplit, l.cout % 8 == 0, l.cin % 8 == 0};
         gemm(ow, l.cout, l.cin, (int)Pn, nkk * nsplit, nsplit, kchunk, stream);
         const int64_t per = (int64_t)l.cout * l.cin;
         hipLaunchKernelGGL(k_sum_parts, dim3(nblk(nkk * per)), dim3(256), 0, stream, part, G + l.woff, nkk, nsplit, per);
         if (gin) {
-            OpConvBwdData od{geom(in.level), gb, Pb + l.woff, gin, l.cin, l.cout, l.ksz, add};
+            OpConvBwdData od{geom(in.level), gb, PbT + l.woff, gin, l.cin, l.cout, l.ksz, add, l.cout % 8 == 0,
+                             l.cout % 8 == 0};
             gemm(od, (int)Pn, l.cin, l.cout * nkk, 1, 1, l.cout * nkk, stream);
         }
     }
@@ -462,8 +568,9 @@ void layers(mts_net_s* n) {
 extern "C" {
 
 int mts_net_create(int c0, int max_batch, int max_h, int max_w, uint64_t seed, mts_net* out) {
-    if (!out || c0 <= 0 || max_batch <= 0 || max_h <= 0 || max_w <= 0 || max_h % 8 || max_w % 8)
-        return set_error(MTS_ERR_ARG, "bad network shape (sizes must be multiples of 8)");
+    // c0 % 8 == 0 keeps every layer's weights 16-byte aligned for the vector loads
+    if (!out || c0 <= 0 || c0 % 8 || max_batch <= 0 || max_h <= 0 || max_w <= 0 || max_h % 8 || max_w % 8)
+        return set_error(MTS_ERR_ARG, "bad network shape (c0 and the sizes must be multiples of 8)");
     *out = nullptr;
     mts_net_s* n = new (std::nothrow) mts_net_s();
     if (!n) return set_error(MTS_ERR_OOM, "host allocation");
@@ -484,7 +591,7 @@ int mts_net_create(int c0, int max_batch, int max_h, int max_w, uint64_t seed, m
     int rc;
     const int64_t np = n->nparam;
     if ((rc = n->alloc(&n->P, np)) || (rc = n->alloc(&n->G, np)) || (rc = n->alloc(&n->Mo, np)) ||
-        (rc = n->alloc(&n->Vo, np)) || (rc = n->alloc(&n->Pb, np)))
+        (rc = n->alloc(&n->Vo, np)) || (rc = n->alloc(&n->Pb, np)) || (rc = n->alloc(&n->PbT, np)))
         return bail(rc);
     // tensors: channels and level
     const int* c = n->c;
@@ -524,7 +631,7 @@ int mts_net_create(int c0, int max_batch, int max_h, int max_w, uint64_t seed, m
         (e = hipMemset(n->Mo, 0, np * 4)) != hipSuccess || (e = hipMemset(n->Vo, 0, np * 4)) != hipSuccess ||
         (e = hipMemset(n->G, 0, np * 4)) != hipSuccess)
         return bail(set_error(MTS_ERR_HIP, std::string("init: ") + hipGetErrorString(e)));
-    hipLaunchKernelGGL(k_f2bf, dim3(nblk(np)), dim3(256), 0, n->stream, n->P, n->Pb, np);
+    n->refresh_bf16();
     if ((e = hipStreamSynchronize(n->stream)) != hipSuccess)
         return bail(set_error(MTS_ERR_HIP, std::string("init: ") + hipGetErrorString(e)));
     *out = n;
@@ -555,7 +662,7 @@ int mts_net_set_params(mts_net n, const float* host) {
     if (int rc = check_net(n)) return rc;
     if (!host) return set_error(MTS_ERR_ARG, "null argument");
     NET_TRY(hipMemcpyAsync(n->P, host, n->nparam * 4, hipMemcpyHostToDevice, n->stream));
-    hipLaunchKernelGGL(k_f2bf, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->P, n->Pb, n->nparam);
+    n->refresh_bf16();
     NET_TRY(hipStreamSynchronize(n->stream));
     return MTS_OK;
 }
@@ -677,7 +784,7 @@ int mts_net_adam(mts_net n, float lr, float beta1, float beta2, float eps, int s
     const float c1 = 1.0f - std::pow(beta1, (float)step), c2 = 1.0f - std::pow(beta2, (float)step);
     hipLaunchKernelGGL(k_adam, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->P, n->G, n->Mo, n->Vo, n->nparam, lr,
                        beta1, beta2, eps, c1, c2);
-    hipLaunchKernelGGL(k_f2bf, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->P, n->Pb, n->nparam);
+    n->refresh_bf16();
     NET_TRY(hipGetLastError());
     NET_TRY(hipStreamSynchronize(n->stream));
     return MTS_OK;
