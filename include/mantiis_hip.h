@@ -313,6 +313,36 @@ int mts_dev_download(void* dst, const void* src, size_t bytes);
  * FETCH_SIZE / WRITE_SIZE counters. */
 int mts_probe_copy(float* dst, const float* src, int64_t n, int launches);
 
+/* ------------------------------------------------------------------------
+ * Geometry preprocessing: replaces Shape::calculateProperties
+ * (include/shape.hpp:127-251 of the reference) -- Zhang-Suen skeleton of the
+ * domain on a canvas with max(nx, ny) pixels of margin (interior rows run on
+ * as fluid), exact distances to the walls and to the skeleton, and per
+ * fluid pixel LocPore = 2 x (wall distance at the first skeleton pixel, in
+ * row-major order, at the skeleton distance), Kn = mfp / LocPore /
+ * resolution.  Wall pixels get 0.  Sides 2..2048 (float distances stay
+ * exact).  Errors: MTS_ERR_ARG for values other than 0 / 1,
+ * MTS_ERR_GEOMETRY when the skeleton misses the domain (the reference then
+ * reads out of bounds, shape.hpp:232-245).
+ * ---------------------------------------------------------------------- */
+typedef struct mts_shape_s* mts_shape;
+/* one-shot form of the reference call: host domain ny x nx row-major int32
+ * (Shape::domain), host outputs ny x nx (Shape::LocPore, Shape::Kn) and,
+ * if non-null, the cropped skeleton (ny x nx bytes) */
+int mts_shape_properties(const int32_t* domain, int nx, int ny, double resolution, double mfp, double* locpore,
+                         double* kn, uint8_t* skeleton);
+/* device-resident form: a handle holds the work buffers of one image size;
+ * mts_shape_run reads a device domain (ny x nx bytes, 0 / 1) and writes
+ * device locpore / kn (and skeleton when non-null) on the handle's stream,
+ * which it synchronizes.  iterations (may be null): full thinning
+ * iterations run on the device, a multiple of 16 no smaller than the
+ * reference's count. */
+int mts_shape_create(int nx, int ny, mts_shape* out);
+int mts_shape_destroy(mts_shape h);
+int mts_shape_run(mts_shape h, const uint8_t* domain, double resolution, double mfp, double* locpore, double* kn,
+                  uint8_t* skeleton, int* iterations);
+int mts_shape_stream(mts_shape h, void** stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,7 +37,8 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_batch_stream", "mts_dev_alloc", "mts_dev_free", "mts_dev_upload", "mts_dev_download",
            "mts_net_create", "mts_net_destroy", "mts_net_param_count", "mts_net_get_params", "mts_net_set_params",
            "mts_net_get_grads", "mts_net_forward", "mts_net_backward", "mts_net_adam", "mts_net_gemm_check",
-           "mts_net_stream"]
+           "mts_net_stream", "mts_shape_create", "mts_shape_destroy", "mts_shape_stream", "mts_shape_run",
+           "mts_shape_properties"]
 
 
 class HipLibraryMissing(RuntimeError):
@@ -531,3 +532,62 @@ def gemm_check(A, B):
     Cm = np.zeros((M, N), np.float32)
     _check(lib, lib.mts_net_gemm_check(A.ctypes.data_as(f), B.ctypes.data_as(f), M, N, K, Cm.ctypes.data_as(f)))
     return Cm
+
+
+def shape_properties(domain, resolution, mfp, skeleton=False):
+    """Shape::calculateProperties (shape.hpp:127-251) on the device, one
+    shot: domain is the row-major [ny][nx] 0/1 image (Shape::domain).
+    Returns (LocPore, Kn) -- and the cropped skeleton when skeleton=True."""
+    lib = load_library()
+    u8 = C.POINTER(C.c_uint8)
+    lib.mts_shape_properties.argtypes = [_I32, C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, u8]
+    d = np.ascontiguousarray(domain, np.int32)
+    ny, nx = d.shape
+    lp = np.zeros((ny, nx))
+    kn = np.zeros((ny, nx))
+    sk = np.zeros((ny, nx), np.uint8) if skeleton else None
+    _check(lib, lib.mts_shape_properties(_p(d, _I32), nx, ny, float(resolution), float(mfp), _p(lp, _D), _p(kn, _D),
+                                         sk.ctypes.data_as(u8) if skeleton else None))
+    return (lp, kn, sk) if skeleton else (lp, kn)
+
+
+class ShapeProperties:
+    """Device-resident calculateProperties for one image size (mts_shape_*):
+    run() reads a DeviceArray domain (uint8 0/1) and writes DeviceArray
+    LocPore / Kn (float64) and optionally the skeleton (uint8)."""
+
+    def __init__(self, nx, ny):
+        self.lib = load_library()
+        for name, args in (("mts_shape_create", [C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+                           ("mts_shape_destroy", [C.c_void_p]),
+                           ("mts_shape_run", [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.POINTER(C.c_int)]),
+                           ("mts_shape_stream", [C.c_void_p, C.POINTER(C.c_void_p)])):
+            getattr(self.lib, name).argtypes = args
+        h = C.c_void_p()
+        _check(self.lib, self.lib.mts_shape_create(int(nx), int(ny), C.byref(h)))
+        self.h = h.value
+        self.nx, self.ny = int(nx), int(ny)
+
+    def run(self, domain, resolution, mfp, locpore, kn, skeleton=None):
+        it = C.c_int(0)
+        _check(self.lib, self.lib.mts_shape_run(self.h, domain.data_ptr(), float(resolution), float(mfp),
+                                                locpore.data_ptr(), kn.data_ptr(),
+                                                skeleton.data_ptr() if skeleton is not None else None, C.byref(it)))
+        return it.value
+
+    def stream(self):
+        s = C.c_void_p()
+        _check(self.lib, self.lib.mts_shape_stream(self.h, C.byref(s)))
+        return s.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mts_shape_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

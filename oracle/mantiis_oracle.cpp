@@ -848,6 +848,7 @@ struct Sim {
 }  // namespace orc
 
 #include "multiblock_oracle.inc"
+#include "shape_oracle.inc"
 
 // ============================================================================
 // C ABI used by the Python tests (ctypes) and by bench.py's CPU leg.
@@ -1104,6 +1105,42 @@ int64_t orc_parent_child(void* h, int64_t* rows) {
     Sim* s = static_cast<Sim*>(h);
     if (rows) std::copy(s->G.parentChild.begin(), s->G.parentChild.end(), rows);
     return (int64_t)s->G.parentChild.size() / 5;
+}
+// Shape::calculateProperties (shape.hpp:127-251).  domain ny x nx row-major;
+// locpore, kn: ny x nx; skeleton (may be null): the cropped skeleton.
+int orc_shape_properties(const int* domain, int nx, int ny, double resolution, double mfp, double* locpore,
+                         double* kn, uint8_t* skeleton, int* iterations) {
+    try {
+        shp::Properties P;
+        shp::calculate(domain, nx, ny, resolution, mfp, P);
+        std::copy(P.locpore.begin(), P.locpore.end(), locpore);
+        std::copy(P.kn.begin(), P.kn.end(), kn);
+        if (skeleton) std::copy(P.skeleton.begin(), P.skeleton.end(), skeleton);
+        if (iterations) *iterations = P.iterations;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+// utils.hpp:518-536 in place on a 0/1 image; returns the iterations run.
+int orc_thinning(uint8_t* img, int rows, int cols) {
+    std::vector<uint8_t> v(img, img + (size_t)rows * cols);
+    const int it = shp::thin(v, rows, cols);
+    std::copy(v.begin(), v.end(), img);
+    return it;
+}
+// Distance of every pixel to the nearest zero pixel of fg: the float value
+// and the exact integer square (-1: no zero pixel).
+void orc_distance(const uint8_t* fg, int rows, int cols, float* value, int64_t* square) {
+    const std::vector<int64_t> k2 = shp::squaredDistance(fg, rows, cols);
+    for (size_t p = 0; p < k2.size(); p++) {
+        if (value) value[p] = shp::distanceValue(k2[p]);
+        if (square) square[p] = k2[p];
+    }
+}
+int64_t orc_nearest_set(const uint8_t* img, int rows, int cols, int i, int j, double D) {
+    return shp::nearestSetPixel(img, rows, cols, i, j, D);
 }
 int orc_threads() {
 #ifdef _OPENMP

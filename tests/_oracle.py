@@ -63,7 +63,58 @@ def load(fast=False):
     lib.orc_parent_child.restype = C.c_int64
     lib.orc_parent_child.argtypes = [_P, _L]
     lib.orc_pixel_domain.argtypes = [_P, _D, _D, _D]
+    _U8 = C.POINTER(C.c_uint8)
+    lib.orc_shape_properties.argtypes = [_I, C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _U8, _I]
+    lib.orc_thinning.argtypes = [_U8, C.c_int, C.c_int]
+    lib.orc_distance.argtypes = [_U8, C.c_int, C.c_int, C.POINTER(C.c_float), _L]
+    lib.orc_nearest_set.restype = C.c_int64
+    lib.orc_nearest_set.argtypes = [_U8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
     return lib
+
+
+def mfp_of(dx=1e-9, T=300.0, P=2e6):
+    """Mean free path (m) of the default gas state (globalParameters.hpp)."""
+    out = np.zeros(10)
+    load().orc_globals_for(dx, T, P, _ptr(out, _D))
+    return float(out[1])
+
+
+def shape_properties(domain, resolution, mfp, fast=False):
+    """Shape::calculateProperties (shape.hpp:127-251) of a row-major 0/1 image.
+    Returns (locpore, kn, skeleton, iterations)."""
+    lib = load(fast)
+    d = np.ascontiguousarray(domain, dtype=np.int32)
+    ny, nx = d.shape
+    lp = np.zeros((ny, nx))
+    kn = np.zeros((ny, nx))
+    sk = np.zeros((ny, nx), np.uint8)
+    it = C.c_int(0)
+    if lib.orc_shape_properties(_ptr(d, _I), nx, ny, resolution, mfp, _ptr(lp, _D), _ptr(kn, _D),
+                                _ptr(sk, C.POINTER(C.c_uint8)), C.byref(it)) != 0:
+        raise RuntimeError(lib.orc_last_error().decode())
+    return lp, kn, sk, it.value
+
+
+def thinning(img):
+    """utils::thinning (utils.hpp:518-536) of a 0/1 image; returns (image, iterations)."""
+    a = np.ascontiguousarray(img, dtype=np.uint8).copy()
+    it = load().orc_thinning(_ptr(a, C.POINTER(C.c_uint8)), a.shape[0], a.shape[1])
+    return a, it
+
+
+def distance(fg):
+    """Exact distance transform to the nearest zero pixel: (float32 values, int64 squares)."""
+    a = np.ascontiguousarray(fg, dtype=np.uint8)
+    v = np.zeros(a.shape, np.float32)
+    k = np.zeros(a.shape, np.int64)
+    load().orc_distance(_ptr(a, C.POINTER(C.c_uint8)), a.shape[0], a.shape[1], _ptr(v, C.POINTER(C.c_float)),
+                        _ptr(k, _L))
+    return v, k
+
+
+def nearest_set(img, i, j, D):
+    a = np.ascontiguousarray(img, dtype=np.uint8)
+    return int(load().orc_nearest_set(_ptr(a, C.POINTER(C.c_uint8)), a.shape[0], a.shape[1], i, j, D))
 
 
 def read_column_major(path, n, dtype=float):
