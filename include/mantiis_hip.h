@@ -113,7 +113,10 @@ int mts_grid_destroy(mts_grid g);
 int mts_grid_size(mts_grid g, int64_t* n, int32_t* max_level);
 /* Grid2D arrays (any pointer may be NULL): gridID [n], gridIJ [n*2] = {x, y},
  * gridLevel [n], gridConnect [n*9], bndTypes [n], gridType [n],
- * gridIsBuffer [n], taus [n] (relaxation time of each cell). */
+ * gridIsBuffer [n], taus [n] (relaxation time of each cell).  The taus of a
+ * multi-block grid come from per-level pore sizes (LB2D::calculateMultigridTauS,
+ * coreLB.hpp:432-480) computed on the current device: only that request
+ * needs a GPU. */
 int mts_grid_arrays(mts_grid g, int64_t* gridID, int32_t* gridIJ, int32_t* level, int64_t* connect, int32_t* bnd,
                     int32_t* type, int32_t* isBuffer, double* taus);
 /* which = 0: Grid2D::gridParentChild rows {cell, tl, tr, bl, br};

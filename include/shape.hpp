@@ -76,10 +76,22 @@ public:
     // One process holds the whole geometry.
     void MPI_broadcast(int = 0) {}
 
-    // Skeleton / distance-transform pore-size estimate (shape.hpp:127-251).
-    void calculateProperties(double, double, bool = false) {
-        throw std::runtime_error("Shape::calculateProperties is not provided by the MI355X build; "
-                                 "load Kn and local pore size with loadExistingModel");
+    // Skeleton / distance-transform pore-size estimate (shape.hpp:127-251),
+    // computed on the GPU: sets resolution, LocPore and Kn of every fluid
+    // pixel (wall pixels keep their values, as in the reference).  The
+    // skeleton window of the reference (dispSkeleton) is not opened.
+    void calculateProperties(double res, double mfp, bool = false) {
+        resolution = res;
+        const std::vector<int32_t> d = flatDomain();
+        std::vector<double> lp((size_t)Nx * Ny), kn((size_t)Nx * Ny);
+        if (mts_shape_properties(d.data(), Nx, Ny, res, mfp, lp.data(), kn.data(), nullptr) != MTS_OK)
+            throw std::runtime_error(mts_last_error());
+        for (int i = 0; i < Ny; i++)
+            for (int j = 0; j < Nx; j++)
+                if (domain[i][j] == 1) {
+                    LocPore[i][j] = lp[(size_t)i * Nx + j];
+                    Kn[i][j] = kn[(size_t)i * Nx + j];
+                }
     }
 
     // row-major flat copies for the C ABI

@@ -482,12 +482,15 @@ static int create_multi32(mts_sim_s* s, const std::vector<CoarseRow>& cr, const 
 static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
                         std::string& err) {
     s->multi = true;
+    hipError_t e = hipSetDevice(s->prm.device);
+    if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
     int rc = build_multi_grid(nx, ny, pore, kn, locpore, s->ph, s->prm.periodic_y != 0, s->mg, err);
+    if (rc) return rc;
+    rc = multi_grid_taus(s->ph, s->mg, err);
     if (rc) return rc;
     rc = build_level_images(s->mg, s->limg, s->mparams, s->ph, err);
     if (rc) return rc;
-    hipError_t e = hipSetDevice(s->prm.device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
     const size_t esz = s->prm.precision == MTS_FP32DEV ? 4 : 8;
     int64_t Pmax = 0;
@@ -894,6 +897,7 @@ struct mts_grid_s {
     bool multi = false;
     Geometry single;
     MultiGeometry mg;
+    Physics ph;
     int64_t n() const { return multi ? mg.ncell : single.ncell; }
 };
 
@@ -923,6 +927,7 @@ int mts_grid_create(int nx, int ny, const int32_t* pore, const double* kn, const
     mts_grid_s* g = new (std::nothrow) mts_grid_s();
     if (!g) return fail(MTS_ERR_OOM, "host allocation");
     std::string err;
+    g->ph = ph;
     const int rc = build_grid(nx, ny, pore, kn, locpore, prm, ph, *g, err);
     if (rc) {
         delete g;
@@ -947,6 +952,11 @@ int mts_grid_size(mts_grid g, int64_t* n, int32_t* max_level) {
 int mts_grid_arrays(mts_grid g, int64_t* gridID, int32_t* gridIJ, int32_t* level, int64_t* connect, int32_t* bnd,
                     int32_t* type, int32_t* isBuffer, double* taus) {
     if (!g) return fail(MTS_ERR_ARG, "null grid");
+    if (taus && g->multi && g->mg.taus.empty()) {
+        std::string err;
+        const int rc = multi_grid_taus(g->ph, g->mg, err);
+        if (rc) return fail(rc, err);
+    }
     const int64_t n = g->n();
     for (int64_t c = 0; c < n; c++) {
         if (g->multi) {

@@ -358,14 +358,63 @@ int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, cons
             G.fineRows.insert(G.fineRows.end(), row, row + 5);
         }
     }
-    // ---- per-cell relaxation (per level, from the loaded maps)
-    G.taus.resize(n);
+    // ---- per-cell Kn (the loaded map at the cell's first pixel); the
+    // relaxation times need the device (multi_grid_taus)
     G.kn.resize(n);
-    for (int64_t c = 0; c < n; c++) {
-        const size_t p = (size_t)G.cy[c] * nx + G.cx[c];
-        const double K = kn[p], Lp = locpore[p] / std::pow(2.0, G.level[c] - 1);
-        G.taus[c] = 0.5 + std::sqrt(6.0 / M_PI) * Lp * K / (1 + 2 * K);
-        G.kn[c] = K;
+    for (int64_t c = 0; c < n; c++) G.kn[c] = kn[(size_t)G.cy[c] * nx + G.cx[c]];
+    G.taus.clear();
+    G.pore.assign(pore, pore + (size_t)nx * ny);
+    (void)locpore;
+    return MTS_OK;
+}
+
+// utils.hpp:539-568 at the sizes the quadtree uses: cv::resize (bilinear)
+// of the 0/255 image to 1/f of its side, then threshold at 128.  Output
+// pixel (i, j) samples source position f (i, j) + (f - 1) / 2, half way
+// between the pixels f/2 - 1 and f/2 of its block in each direction, so its
+// value is the mean of those 2 x 2 pixels rounded to 8 bits (0, 64, 128,
+// 191, 255): fluid when at least three of the four are fluid.  f = 1 is
+// the image itself.
+static int32_t resized_pixel(const int32_t* pore, int nx, int f, int i, int j) {
+    if (f == 1) return pore[(size_t)i * nx + j] != 0;
+    const int r0 = f * i + f / 2 - 1, c0 = f * j + f / 2 - 1;
+    const int n = (pore[(size_t)r0 * nx + c0] != 0) + (pore[(size_t)r0 * nx + c0 + 1] != 0) +
+                  (pore[(size_t)(r0 + 1) * nx + c0] != 0) + (pore[(size_t)(r0 + 1) * nx + c0 + 1] != 0);
+    return n >= 3;
+}
+
+// LB2D::calculateMultigridTauS (coreLB.hpp:432-480): for each level
+// lvl = 0..maxLevel-1 the domain resized by 2^-lvl, with wall rows first and
+// last (Shape::addHorizontalBoundary), goes through calculateProperties at
+// resolution Cl 2^lvl (on the device, mts_shape_properties); the level's
+// map is tau = 0.5 + sqrt(6/pi) LocPore Kn / (1 + 2 Kn) (0.5 at walls), and
+// a cell of level L takes its map's value at (y, x) / 2^(L-1).
+int multi_grid_taus(const Physics& ph, MultiGeometry& G, std::string& err) {
+    const int nx = G.nx, ny = G.ny;
+    if (G.pore.size() != (size_t)nx * ny) {
+        err = "multi-block grid without its pore image";
+        return MTS_ERR_STATE;
+    }
+    std::vector<std::vector<double>> tauMap(G.maxLevel);
+    for (int lvl = 0; lvl < G.maxLevel; lvl++) {
+        const int f = 1 << lvl, lx = nx / f, ly = ny / f;
+        std::vector<int32_t> dom((size_t)lx * ly, 0);
+        for (int i = 1; i < ly - 1; i++)
+            for (int j = 0; j < lx; j++) dom[(size_t)i * lx + j] = resized_pixel(G.pore.data(), nx, f, i, j);
+        std::vector<double> lp((size_t)lx * ly), kn((size_t)lx * ly);
+        const int rc = mts_shape_properties(dom.data(), lx, ly, ph.Cl * f, ph.mfp, lp.data(), kn.data(), nullptr);
+        if (rc != MTS_OK) {
+            err = std::string("per-level pore size (level ") + std::to_string(lvl + 1) + "): " + mts_last_error();
+            return rc;
+        }
+        std::vector<double>& T = tauMap[lvl];
+        T.resize((size_t)lx * ly);
+        for (size_t p = 0; p < T.size(); p++) T[p] = 0.5 + std::sqrt(6.0 / M_PI) * lp[p] * kn[p] / (1 + 2 * kn[p]);
+    }
+    G.taus.resize(G.ncell);
+    for (int64_t c = 0; c < G.ncell; c++) {
+        const int L = G.level[c], f = 1 << (L - 1), lx = nx / f;
+        G.taus[c] = tauMap[L - 1][(size_t)(G.cy[c] / f) * lx + G.cx[c] / f];
     }
     return MTS_OK;
 }

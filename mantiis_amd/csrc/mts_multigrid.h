@@ -17,7 +17,8 @@ struct MultiGeometry {
     // pixel), then buffer cells in creation order
     std::vector<int32_t> cx, cy, level, type, isBuffer, bnd;
     std::vector<int64_t> gridID, connect;   // connect[c*9+k]: cell, -1 wall, -2 missing
-    std::vector<double> taus, kn;
+    std::vector<double> taus, kn;           // taus: empty until multi_grid_taus
+    std::vector<int32_t> pore;              // the pore image, row-major
     std::vector<int64_t> parentChild;       // rows {cell, tl, tr, bl, br}
     std::vector<int64_t> coarseRows;        // rows {coarse buffer, 4 children} (type 21)
     std::vector<int64_t> fineRows;          // rows {parent, 4 fine buffer cells} (type 12)
@@ -25,6 +26,8 @@ struct MultiGeometry {
 
 int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore, const Physics& ph,
                      bool torus, MultiGeometry& G, std::string& err);
+// Per-cell relaxation times of a built grid (needs the current device).
+int multi_grid_taus(const Physics& ph, MultiGeometry& G, std::string& err);
 
 }  // namespace mts
 

@@ -310,7 +310,9 @@ class HostGrid:
         _check(self.lib, self.lib.mts_grid_size(self.h, C.byref(n), C.byref(L)))
         return n.value, L.value
 
-    def arrays(self):
+    def arrays(self, taus=True):
+        """Grid2D arrays; taus=False leaves out the relaxation times, which a
+        multi-block grid computes on the device."""
         n, _ = self.size()
         gid = np.zeros(n, np.int64)
         ij = np.zeros((n, 2), np.int32)
@@ -319,10 +321,14 @@ class HostGrid:
         bnd = np.zeros(n, np.int32)
         typ = np.zeros(n, np.int32)
         isb = np.zeros(n, np.int32)
-        taus = np.zeros(n)
+        t = np.zeros(n) if taus else None
         _check(self.lib, self.lib.mts_grid_arrays(self.h, _p(gid, _I64), _p(ij, _I32), _p(lvl, _I32), _p(con, _I64),
-                                                  _p(bnd, _I32), _p(typ, _I32), _p(isb, _I32), _p(taus, _D)))
-        return dict(gridID=gid, gridIJ=ij, level=lvl, connect=con, bnd=bnd, type=typ, isBuffer=isb, taus=taus)
+                                                  _p(bnd, _I32), _p(typ, _I32), _p(isb, _I32),
+                                                  _p(t, _D) if taus else None))
+        out = dict(gridID=gid, gridIJ=ij, level=lvl, connect=con, bnd=bnd, type=typ, isBuffer=isb)
+        if taus:
+            out["taus"] = t
+        return out
 
     def rows(self, which):
         w = {"parent_child": 0, "coarse": 1, "fine": 2}[which]

@@ -847,8 +847,8 @@ struct Sim {
 
 }  // namespace orc
 
-#include "multiblock_oracle.inc"
 #include "shape_oracle.inc"
+#include "multiblock_oracle.inc"
 
 // ============================================================================
 // C ABI used by the Python tests (ctypes) and by bench.py's CPU leg.

@@ -66,3 +66,21 @@ def test_method_by_method_driver(tmp_path):
         assert np.abs(a - g).max() <= 1e-10, name
     conv = np.loadtxt(folder + "m_convergence.txt")
     assert abs(conv[0] - 0.466825156246162) < 1e-13 and abs(conv[99] - 0.00151596085691759) < 1e-14
+
+
+def test_reference_style_calculate_properties(tmp_path):
+    """Shape::calculateProperties through include/shape.hpp on the fixture's
+    pore image: the maps written by Shape::writeToText (15 digits) equal the
+    oracle's."""
+    banner("drop-in Shape::calculateProperties 64x64")
+    exe = build("shape_driver", tmp_path)
+    folder = fixture_copy(tmp_path)
+    out = tmp_path / "out"
+    out.mkdir()
+    subprocess.run([str(exe), folder, str(out) + "/", "64"], check=True, timeout=120)
+    pore, _, _ = O.load_fixture_geometry()
+    lp, kn, _, _ = O.shape_properties(pore, 1e-9, O.mfp_of())
+    glp = np.loadtxt(out / "localporesize.dat").reshape(64, 64)
+    gkn = np.loadtxt(out / "Kn.dat").reshape(64, 64)
+    assert np.allclose(glp, lp, rtol=1e-14, atol=0)
+    assert np.allclose(gkn, kn, rtol=1e-14, atol=0)

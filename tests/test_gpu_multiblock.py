@@ -102,3 +102,33 @@ def test_multiblock_fp32_call_chunking_is_exact():
     d = np.abs(a.get("f") - b.get("f")).max()
     print("max |f| diff", d)
     assert d <= 1e-14
+
+
+def test_multiblock_taus_per_level():
+    """LB2D::calculateMultigridTauS (coreLB.hpp:432-480): every cell's tau
+    from its level's resized image and calculateProperties, through the
+    device lattice and the host grid, equals the oracle's; a straight
+    channel gives the closed form on both levels."""
+    banner("multi-block per-level taus vs oracle (box64, disks128, channel64)")
+    from mantiis_amd.hip import HostGrid
+    N = 64
+    chan = np.ones((N, N), np.int32)
+    chan[0] = chan[-1] = 0
+    for name, pore, torus in (("box64", box64(), False), ("disks128", G.disk_packing(128, 0.7, 3, 8, seed=3), True),
+                              ("channel64", chan, False)):
+        kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+        o = O.Oracle(pore, kn, lp, multiblock=True, periodic_y=torus)
+        g = Lattice(pore, kn, lp, precision="fp32dev", multiblock=1, periodic_y=int(torus))
+        h = HostGrid(pore, kn, lp, multiblock=1, periodic_y=int(torus))
+        t = o.get("taus")
+        assert np.array_equal(g.get("taus"), t), name
+        assert np.array_equal(h.arrays()["taus"], t), name
+        if name == "channel64":
+            lvl = h.arrays()["level"]
+            mfp = O.mfp_of()
+            for L, side in ((1, N), (2, N // 2)):
+                lpL = 2 * ((side - 1) // 2)  # middle row of the side-row channel, doubled
+                knL = mfp / lpL / (1e-9 * 2 ** (L - 1))
+                want = 0.5 + np.sqrt(6 / np.pi) * lpL * knL / (1 + 2 * knL)
+                assert (lvl == L).any() and np.allclose(t[lvl == L], want, rtol=1e-14), L
+        g.close()

@@ -43,10 +43,13 @@ def test_library_grid_equals_oracle(geom, multiblock, torus):
     g = HostGrid(pore, kn, lp, multiblock=int(multiblock), periodic_y=int(torus))
     n, L = g.size()
     assert n == o.ncell and L == o.lib.orc_max_level(o.h)
-    a, b = o.grid(), g.arrays()
+    # a multi-block grid's relaxation times come from per-level pore sizes
+    # computed on the device: tests/test_gpu_multiblock.py compares them
+    a, b = o.grid(), g.arrays(taus=not multiblock)
     for k in ("gridID", "gridIJ", "level", "connect", "bnd", "isBuffer", "type"):
         assert np.array_equal(a[k], b[k]), k
-    assert np.array_equal(o.get("taus"), b["taus"])
+    if not multiblock:
+        assert np.array_equal(o.get("taus"), b["taus"])
     if multiblock:
         for which in ("parent_child", "coarse", "fine"):
             assert np.array_equal(O.oracle_rows(o, which), g.rows(which)), which
@@ -61,7 +64,7 @@ def test_multiblock_structure_invariants():
     pore = G.disk_packing(128, 0.7, 3, 8, seed=3)
     kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
     g = HostGrid(pore, kn, lp, multiblock=1, periodic_y=1)
-    a = g.arrays()
+    a = g.arrays(taus=False)
     walls = np.argwhere(pore == 0)
     N = pore.shape[0]
     for c in np.nonzero((a["level"] == 2) & (a["isBuffer"] == 0))[0][:200]:

@@ -162,3 +162,27 @@ def test_rejects_non_binary():
     d[3, 3] = 2
     with pytest.raises(RuntimeError):
         O.shape_properties(d, 1e-9, O.mfp_of())
+
+
+def test_multiblock_taus_channel_closed_form():
+    """LB2D::calculateMultigridTauS (coreLB.hpp:432-480) in the oracle: in a
+    straight channel (walls on rows 0 and N-1) level L sees a channel of
+    N / 2^(L-1) rows with forced wall rows, so LocPore_L = 2 floor((side-1)/2)
+    at resolution 1e-9 2^(L-1), and every level-L cell has
+    tau = 0.5 + sqrt(6/pi) LocPore Kn / (1 + 2 Kn)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from mantiis_amd import geometry as G
+    N = 64
+    chan = np.ones((N, N), np.int32)
+    chan[0] = chan[-1] = 0
+    kn, lp = G.uniform_kn(chan, 0.1, 0.7730846760658538)
+    o = O.Oracle(chan, kn, lp, multiblock=True)
+    t, lvl = o.get("taus"), o.grid()["level"]
+    mfp = O.mfp_of()
+    for L, side in ((1, N), (2, N // 2)):
+        lpL = 2 * ((side - 1) // 2)
+        knL = mfp / lpL / (1e-9 * 2 ** (L - 1))
+        assert (lvl == L).any()
+        assert np.allclose(t[lvl == L], 0.5 + np.sqrt(6 / np.pi) * lpL * knL / (1 + 2 * knL), rtol=1e-14)
