@@ -350,12 +350,80 @@ def bench_c5(args):
     return 0
 
 
+def bench_shape(args):
+    """Shape::calculateProperties (shape.hpp:127-251) on the device: LocPore
+    and Kn of an n x n disk packing, domain and outputs resident in HBM.  A
+    step is one full run (canvas, thinning to convergence, two distance
+    transforms, nearest-skeleton search)."""
+    from mantiis_amd import geometry as G
+    from mantiis_amd.hip import DeviceArray, ShapeProperties
+    from mantiis_amd.hiptime import StreamTimer
+    n = args.n or 1024
+    pore = G.disk_packing(n, 0.6, 3, 16, seed=31)
+    mfp = 0.7730846760658538e-9
+    h = ShapeProperties(n, n)
+    d = DeviceArray.from_numpy(pore.astype(np.uint8))
+    lp, kn = DeviceArray((n, n), np.float64), DeviceArray((n, n), np.float64)
+    for _ in range(args.warmup):
+        it = h.run(d, 1e-9, mfp, lp, kn)
+    timer = StreamTimer(h.stream())
+    t0 = time.perf_counter()
+    timer.start()
+    for _ in range(args.steps):
+        it = h.run(d, 1e-9, mfp, lp, kn)
+    ms = timer.stop()
+    wall = time.perf_counter() - t0
+    timer.close()
+    pix = n * n
+    value = pix * args.steps / (ms / 1e3) / 1e6
+    bytes_per_pixel = 17  # domain byte in, LocPore and Kn (FP64) out
+    achieved = bytes_per_pixel * pix * args.steps / (ms / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        cn = args.cpu_n if args.cpu_n != 256 else n
+        cpore = pore if cn == n else G.disk_packing(cn, 0.6, 3, 16, seed=31)
+        t1 = time.perf_counter()
+        O.shape_properties(cpore, 1e-9, mfp, fast=True)
+        cs = time.perf_counter() - t1
+        cpu = {"value": round(cn * cn / cs / 1e6, 4), "unit": "Mpix/s", "cores": 1, "kind": "port",
+               "sample": f"one calculateProperties of a {cn}x{cn} disk packing by the oracle (-Ofast, 1 thread), "
+                         f"{cs:.2f} s"}
+    out = {
+        "metric": "Mpix/s (calculateProperties)",
+        "value": round(value, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 bit planes (thinning) / int32 squares (distances) / fp64 (LocPore, Kn)",
+        "data": "synthetic (seeded disk packing)",
+        "config": {"workload": f"Shape::calculateProperties of a {n}x{n} disk packing (porosity 0.6, radius "
+                               f"U[3,16]); canvas {3 * n}x{3 * n}, {it} thinning iterations on the device",
+                   "lattice": [n, n], "parallelism": "single GPU", "wall_ms_per_step": round(wall * 1e3 / args.steps, 4)},
+        "roofline": {"bound": "hbm", "kernel": "whole run (k_thin is ~80% of it: a chain of 2 x iterations dependent "
+                                               "subiterations, latency- not bandwidth-bound)",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "bytes_per_pixel": bytes_per_pixel, "pixels_per_run": pix},
+        "cpu_baseline": cpu,
+    }
+    h.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "transport", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "transport", "c1", "shape"])
     ap.add_argument("--train", type=int, default=2000, help="c5: training lattices (256^2)")
     ap.add_argument("--eval", type=int, default=64, help="c5: unseen evaluation lattices (512^2)")
     ap.add_argument("--epochs", type=int, default=20, help="c5: training epochs")
@@ -397,6 +465,8 @@ def main():
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "shape":
+        return bench_shape(args)
     if args.workload == "c3":
         lat, inputs = make_c3()
         kind = "adsorption"
