@@ -186,6 +186,24 @@ public:
         return out;
     }
 
+    // Inverse of prepareDistributions (no reference counterpart): sets f from
+    // an NX*NY*9 pixel-major vector (a dump of f.txt / f.bin) and recomputes
+    // the moments from it (the loops take usq_old, rho_old from them), which
+    // is the whole state a step continues from -- so a run resumed from a
+    // lossless dump continues exactly as the uninterrupted one.  Single-block
+    // lattices only (a multi-block dump keeps one cell per pixel).
+    void loadDistributions(const std::vector<double>& fGlobal) {
+        if (grid.maxLevel > 1) throw std::invalid_argument("loadDistributions: single-block lattices only");
+        if (fGlobal.size() != (size_t)NX * NY * NC) throw std::invalid_argument("loadDistributions: expected NX*NY*9 values");
+        std::vector<double> fc(grid.gridIJ.size() * NC);
+        for (size_t c = 0; c < grid.gridIJ.size(); c++) {
+            const size_t p = (size_t)grid.gridIJ[c][1] * NX + grid.gridIJ[c][0];
+            for (int ic = 0; ic < NC; ic++) fc[c * NC + ic] = fGlobal[p * NC + ic];
+        }
+        f = fc;
+        calculateMacroscopicPorperties();  // rho, ux, uy, usq of the loaded f
+    }
+
 private:
     std::shared_ptr<mts_sim_s> h_;
     uint64_t version_ = 0;

@@ -12,6 +12,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "comm.hpp"
@@ -36,6 +37,17 @@ struct Settings {
     // Steps fused between two residual checks (no reference counterpart; 1
     // reproduces the reference's per-step residual and break test exactly).
     int check_every = 1;
+    // No reference counterpart: start from the populations of a dump
+    // (f.txt, or f.bin written with binary_output) instead of rest -- a resume
+    // or a warm start -- and also write the final fields as raw binary.
+    std::string initial_distributions = "";
+    bool binary_output = false;
+};
+
+// mantiisApp.hpp:66-73: dumps are written by threads while the run goes on.
+template <typename T>
+struct WriterThread {
+    static void write(const std::string& fName, const std::vector<T>& data) { IO::writeVectorToFile(fName, data); }
 };
 
 class MantiisApp {
@@ -55,6 +67,9 @@ private:
     int dump_distributions_first_timestep = 0;
     int dump_macroscopic_every_timestep = 0;
     int check_every = 1;
+    std::string initial_distributions;
+    bool binary_output = false;
+    std::vector<std::thread> writer_threads;
     int proc_id = 0;
     int num_procs = 1;
     int num_threads = 1;
@@ -77,6 +92,8 @@ public:
         dump_macroscopic_every_timestep = settings.dump_macroscopic_every_timestep;
         tol = settings.threshold;
         check_every = settings.check_every > 0 ? settings.check_every : 1;
+        initial_distributions = settings.initial_distributions;
+        binary_output = settings.binary_output;
         _GLOBAL_::mfp = settings.mfp;
         _GLOBAL_::Fbody = settings.Fbody;
         _GLOBAL_::Cl = settings.dx;
@@ -135,12 +152,40 @@ public:
                     "\t 2 for SRBBWall\n\t 1 for MDBBWall\n\t 0 for BBWall\n");
         }
         lb->setOpenBoundary(&LB2D::periodicBoundary);
+        if (!initial_distributions.empty()) {
+            std::vector<double> f0((size_t)Nx * Ny * 9);
+            const std::string& p = initial_distributions;
+            if (p.size() > 4 && p.compare(p.size() - 4, 4, ".bin") == 0) IO::loadBinary(p, f0);
+            else IO::loadDataFromFile(p, f0);
+            lb->loadDistributions(f0);
+        }
     }
+
+    ~MantiisApp() { join_writers(); }
 
     // mantiisApp.hpp:239-284 as fused steps
     void transport_simulation_loop(int n = 1) { lb->stepTransport(n); }
     void adsorption_simulation_loop(int n = 1) { lb->stepAdsorption(n); }
     void coupled_simulation_loop(int n = 1) { lb->stepCoupled(n); }
+
+    // mantiisApp.hpp:309-320, 334-348: finish the earlier writes, then hand
+    // this dump's vectors to new threads
+    void join_writers() {
+        for (auto& th : writer_threads)
+            if (th.joinable()) th.join();
+        writer_threads.clear();
+    }
+    void write_async(const std::string& name, std::vector<double>&& v) {
+        for (auto it = writer_threads.begin(); it != writer_threads.end();) {
+            if (it->joinable()) {
+                it->join();
+                it = writer_threads.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        writer_threads.emplace_back(&WriterThread<double>::write, name, std::move(v));
+    }
 
     // mantiisApp.hpp:286-430
     void run() {
@@ -156,14 +201,15 @@ public:
                     std::cout << "Timestep: " << lb->t << ", residual difference: " << lb->diff << "\n";
                 if (lb->diff < tol) break;
                 if (lb->t < dump_distributions_first_timestep)
-                    IO::writeVectorToFile(folder + "f_" + std::to_string(lb->t) + ".txt", lb->prepareDistributions());
+                    write_async(folder + "f_" + std::to_string(lb->t) + ".txt", lb->prepareDistributions());
                 if (dump_macroscopic_every_timestep > 0 && lb->t % dump_macroscopic_every_timestep == 0) {
                     auto d = lb->completeSingleGridDomain();
-                    IO::writeVectorToFile(folder + "ux_" + std::to_string(lb->t) + ".txt", std::get<0>(d));
-                    IO::writeVectorToFile(folder + "uy_" + std::to_string(lb->t) + ".txt", std::get<1>(d));
+                    write_async(folder + "ux_" + std::to_string(lb->t) + ".txt", std::move(std::get<0>(d)));
+                    write_async(folder + "uy_" + std::to_string(lb->t) + ".txt", std::move(std::get<1>(d)));
                 }
                 lb->t++;
             }
+            join_writers();
         };
         if (simulation_type == "transport") {
             run_loop([&](int n) { transport_simulation_loop(n); });
@@ -185,11 +231,18 @@ public:
         }
         if (dump_final_results) {
             auto d = lb->completeSingleGridDomain();
-            IO::writeVectorToFile(folder + "f.txt", lb->prepareDistributions());
+            const std::vector<double> fd = lb->prepareDistributions();
+            IO::writeVectorToFile(folder + "f.txt", fd);
             IO::writeVectorToFile(folder + "ux.txt", std::get<0>(d));
             IO::writeVectorToFile(folder + "uy.txt", std::get<1>(d));
             IO::writeVectorToFile(folder + "rho.txt", std::get<2>(d));
             IO::writeVectorToFile(folder + "convergence.txt", lb->diff_over_time);
+            if (binary_output) {
+                IO::writeBinary(folder + "f.bin", fd);
+                IO::writeBinary(folder + "ux.bin", std::get<0>(d));
+                IO::writeBinary(folder + "uy.bin", std::get<1>(d));
+                IO::writeBinary(folder + "rho.bin", std::get<2>(d));
+            }
             std::ofstream details(folder + "SimulationDetails.txt");
             details << "Domain dimension: " << shape->Nx << " x " << shape->Ny << "\n";
             details << "Tolerance was set to " << tol << "\n";
