@@ -236,6 +236,7 @@ def bench_c4(args):
     r = b.run(400000, check_every=100, tol=1e-6, graph=True)
     ms = t.stop()
     t.close()
+    thr, xrun, tab = b.tune(-1)  # the configuration the run's autotuner chose
     mlups = r["updates"] / (ms / 1e3) / 1e6
     achieved = 72 * r["updates"] / (ms / 1e3) / 1e9
     conv = r["steps"] > 0
@@ -268,6 +269,10 @@ def bench_c4(args):
         "cpu_baseline": None,
         "batch": {"converged": int(conv.sum()), "run_s": round(ms / 1e3, 3), "cell_updates": int(r["updates"]),
                   "generation_s": round(t_gen, 1), "setup_s": round(t_setup, 1)},
+        "autotune": {"threads_per_block": thr, "xcd_run": xrun,
+                     "ms_per_step": [[round(v, 4) for v in row] for row in tab.tolist()],
+                     "candidates": "rows: threads per 256-cell block 64, 128, 256; columns: XCD run 0, 8, 32; "
+                                   "timed inside the run (8 steps each from its first state, state restored)"},
     }
     print(json.dumps(out), flush=True)
     return 0

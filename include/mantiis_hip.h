@@ -272,6 +272,20 @@ int mts_batch_fields(mts_batch b, int g, double* rho, double* ux, double* uy);
  * f[j*nx*ny + site]; solids ignored): a warm start */
 int mts_batch_set_state(mts_batch b, int g, const double* f);
 int mts_batch_stream(mts_batch b, void** stream);
+/* Autotuner (no reference counterpart): steps_each > 0 times every launch
+ * configuration of the batched step -- threads per 256-cell block {64, 128,
+ * 256} (each thread then takes 4, 2 or 1 cells) x XCD run length of the
+ * queue map {0 (queue order), 8, 32} -- for steps_each steps from the current
+ * state, keeps the fastest and puts the state back (the configuration
+ * changes no result).  steps_each == 0 takes the default (256, 32) untimed;
+ * steps_each < 0 only reports.  mts_batch_run tunes with 8 steps on its first
+ * call if this was never called.  Outputs (may be null): the configuration
+ * and ms_per_step[9] per candidate (row-major threads x run length, -1: not
+ * timed). */
+int mts_batch_tune(mts_batch b, int steps_each, int* threads, int* xcd_run, double* ms_per_step);
+/* Sets the launch configuration by hand (threads 64 / 128 / 256, XCD run
+ * length >= 0) and marks the batch tuned. */
+int mts_batch_set_config(mts_batch b, int threads, int xcd_run);
 
 /* ------------------------------------------------------------------------
  * Learned warm start (no reference counterpart): a four-level convolutional

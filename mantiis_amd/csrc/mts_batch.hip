@@ -58,6 +58,12 @@ struct mts_batch_s {
     std::vector<char> active;
     std::vector<double> u_prev;
     std::vector<void*> allocs;
+    // launch configuration of k_cbat (chosen by tune()): threads per
+    // 256-cell block and XCD run length of the queue map
+    int threads = 256, xrun = 32;
+    bool tuned = false;
+    std::vector<double> tune_ms;  // per candidate: ms per step (-1: not timed)
+    float* snap = nullptr;        // carried state saved while tuning
     // graph of one chunk of steps (replayed while the chunk length holds)
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -120,6 +126,8 @@ struct mts_batch_s {
         L.mac = stats ? mac : nullptr;
         L.part = stats ? part : nullptr;
         L.from_rest = from;
+        L.threads = threads;
+        L.xrun = xrun;
         if (from == 2) {  // the given state sits in A[cur^1]: read it, write A[cur]
             L.Ain = A[cur ^ 1];
             L.Aout = A[cur];
@@ -134,6 +142,64 @@ struct mts_batch_s {
         int rc;
         for (int i = 0; i < n; i++)
             if ((rc = step(i == n - 1, 0))) return rc;
+        return MTS_OK;
+    }
+    // Autotuner: times every launch configuration (threads per block x XCD
+    // run length) for `each` steps from the current state and keeps the
+    // fastest.  The configuration changes no result, and the state is saved
+    // before and put back after, so tuning leaves the simulation where it
+    // was.  each == 0 takes the default configuration without timing.
+    static constexpr int NCAND = 9;
+    static int cand_threads(int k) { return k / 3 == 0 ? 64 : (k / 3 == 1 ? 128 : 256); }
+    static int cand_xrun(int k) { return k % 3 == 0 ? 0 : (k % 3 == 1 ? 8 : 32); }
+    int tune(int each) {
+        tune_ms.assign(NCAND, -1.0);
+        tuned = true;
+        if (each <= 0 || nactive == 0) {
+            threads = 256;
+            xrun = 32;
+            return MTS_OK;
+        }
+        each += each & 1;
+        const size_t elems = (size_t)9 * dev.cstride;
+        if (!snap) {
+            if (int rc = alloc(&snap, elems)) return rc;
+        }
+        const int cur0 = cur;
+        BT_TRY(hipMemcpyAsync(snap, A[cur0], elems * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        BT_TRY(hipEventCreate(&e0));
+        BT_TRY(hipEventCreate(&e1));
+        int best = -1;
+        int rc = MTS_OK;
+        for (int k = 0; k < NCAND && rc == MTS_OK; k++) {
+            threads = cand_threads(k);
+            xrun = cand_xrun(k);
+            for (int i = 0; i < 2 && rc == MTS_OK; i++) rc = step(false, 0);  // first use of the variant
+            if (rc) break;
+            if (hipEventRecord(e0, stream) != hipSuccess) rc = set_error(MTS_ERR_HIP, "tune: event");
+            for (int i = 0; i < each && rc == MTS_OK; i++) rc = step(false, 0);
+            if (rc) break;
+            float ms = 0.0f;
+            if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                rc = set_error(MTS_ERR_HIP, "tune: event timing");
+            tune_ms[k] = (double)ms / each;
+            if (best < 0 || tune_ms[k] < tune_ms[best]) best = k;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        threads = cand_threads(best);
+        xrun = cand_xrun(best);
+        BT_TRY(hipMemcpyAsync(A[cur0], snap, elems * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        BT_TRY(hipStreamSynchronize(stream));
+        cur = cur0;
+        // a captured chunk holds the old configuration
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
         return MTS_OK;
     }
     int chunk_graph(int n) {
@@ -365,6 +431,7 @@ int mts_batch_run(mts_batch b, int64_t max_steps, int check_every, double tol, i
     // baseline mean velocity of the current state
     for (int g = 0; g < nb; g++)
         if (!b->active[g]) st[g] = 0;
+    if (!b->tuned && (rc = b->tune(8))) return rc;
     const auto t0 = std::chrono::steady_clock::now();
     bool first = true;
     while (done < max_steps && b->nactive > 0) {
@@ -403,6 +470,33 @@ int mts_batch_run(mts_batch b, int64_t max_steps, int check_every, double tol, i
         if (ux_mean) ux_mean[g] = b->active[g] || st[g] > 0 ? um[g] : b->u_prev[g];
     }
     if (cell_updates) *cell_updates = upd;
+    return MTS_OK;
+}
+
+int mts_batch_tune(mts_batch b, int steps_each, int* threads, int* xcd_run, double* ms_per_step) {
+    if (int rc = check(b)) return rc;
+    if (steps_each >= 0) {
+        if (int rc = b->tune(steps_each)) return rc;
+    }
+    if (threads) *threads = b->threads;
+    if (xcd_run) *xcd_run = b->xrun;
+    if (ms_per_step)
+        for (int k = 0; k < mts_batch_s::NCAND; k++)
+            ms_per_step[k] = k < (int)b->tune_ms.size() ? b->tune_ms[k] : -1.0;
+    return MTS_OK;
+}
+
+int mts_batch_set_config(mts_batch b, int threads, int xcd_run) {
+    if (int rc = check(b)) return rc;
+    if ((threads != 64 && threads != 128 && threads != 256) || xcd_run < 0)
+        return set_error(MTS_ERR_ARG, "threads must be 64, 128 or 256 and xcd_run >= 0");
+    b->threads = threads;
+    b->xrun = xcd_run;
+    b->tuned = true;
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
     return MTS_OK;
 }
 

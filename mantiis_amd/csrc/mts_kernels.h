@@ -123,6 +123,8 @@ struct BatchLaunch {
     float* mac = nullptr;
     double* part = nullptr;  // per-block (sum rho, sum j_x)
     int from_rest = 0;
+    int threads = 256;       // threads per 256-cell block: 64, 128 or 256
+    int xrun = 32;           // XCD run length of the queue map (0: none)
 };
 int launch_cbatch(const BatchLaunch& L);
 // acc[2g..2g+1] = sum of part over the blocks [first[g], first[g+1])

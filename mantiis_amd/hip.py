@@ -34,7 +34,7 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows",
            "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy", "mts_batch_create", "mts_batch_destroy",
            "mts_batch_cells", "mts_batch_run", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
-           "mts_batch_stream", "mts_dev_alloc", "mts_dev_free", "mts_dev_upload", "mts_dev_download",
+           "mts_batch_stream", "mts_batch_tune", "mts_batch_set_config", "mts_dev_alloc", "mts_dev_free", "mts_dev_upload", "mts_dev_download",
            "mts_net_create", "mts_net_destroy", "mts_net_param_count", "mts_net_get_params", "mts_net_set_params",
            "mts_net_get_grads", "mts_net_forward", "mts_net_backward", "mts_net_adam", "mts_net_gemm_check",
            "mts_net_stream", "mts_shape_create", "mts_shape_destroy", "mts_shape_stream", "mts_shape_run",
@@ -115,6 +115,8 @@ def load_library(path=LIB_PATH):
     lib.mts_batch_fields.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
     lib.mts_batch_set_state.argtypes = [C.c_void_p, C.c_int, _D]
     lib.mts_batch_stream.argtypes = [C.c_void_p, _PP]
+    lib.mts_batch_tune.argtypes = [C.c_void_p, C.c_int, _I32, _I32, _D]
+    lib.mts_batch_set_config.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mts_dev_alloc.argtypes = [C.c_size_t, _PP]
     lib.mts_dev_free.argtypes = [C.c_void_p]
     lib.mts_dev_upload.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
@@ -413,6 +415,19 @@ class Batch:
         s = C.c_void_p()
         _check(self.lib, self.lib.mts_batch_stream(self.h, C.byref(s)))
         return s.value
+
+    def tune(self, steps_each=8):
+        """Autotuner (mts_batch_tune): steps_each > 0 times the launch
+        configurations and keeps the fastest, 0 takes the default, < 0 only
+        reports.  Returns (threads per block, XCD run length, ms per step of
+        each candidate as a 3 x 3 array: threads {64,128,256} x run {0,8,32})."""
+        t, k = C.c_int32(0), C.c_int32(0)
+        ms = np.full(9, -1.0)
+        _check(self.lib, self.lib.mts_batch_tune(self.h, int(steps_each), C.byref(t), C.byref(k), _p(ms, _D)))
+        return t.value, k.value, ms.reshape(3, 3)
+
+    def set_config(self, threads, xcd_run):
+        _check(self.lib, self.lib.mts_batch_set_config(self.h, int(threads), int(xcd_run)))
 
 
 class DeviceArray:

@@ -91,3 +91,30 @@ def test_batch_warm_start_from_converged_state():
     r = c.run(200000, check_every=100, tol=1e-6)
     print("cold", cold, "warm", r["steps"])
     assert (r["steps"] > 0).all() and (r["steps"] < cold).all()
+
+
+def test_batch_launch_configurations_agree_and_tuning_is_invisible():
+    """Every launch configuration of the autotuner (threads per block x XCD
+    run length) gives the same fields bit for bit, and tuning puts the state
+    back: a tuned batch and an untuned one agree after the same run."""
+    banner("batch autotuner: 9 configurations agree; tuning leaves the state")
+    gs = geoms()
+    ref = Batch(gs, **PARAMS)
+    ref.set_config(256, 32)
+    ref.run(200, check_every=100, tol=1e-300, graph=False)
+    want = [ref.fields(g) for g in range(len(gs))]
+    for t in (64, 128, 256):
+        for k in (0, 8, 32):
+            b = Batch(gs, **PARAMS)
+            b.set_config(t, k)
+            b.run(200, check_every=100, tol=1e-300, graph=(t == 128))
+            got = [b.fields(g) for g in range(len(gs))]
+            assert all(np.array_equal(x, y) for fa, fb in zip(got, want) for x, y in zip(fa, fb)), (t, k)
+            b.close()
+    tb = Batch(gs, **PARAMS)
+    t, k, ms = tb.tune(6)
+    print("tuned", t, k, "\n", ms)
+    assert (ms > 0).all() and ms[[64, 128, 256].index(t), [0, 8, 32].index(k)] == ms.min()
+    tb.run(200, check_every=100, tol=1e-300)
+    got = [tb.fields(g) for g in range(len(gs))]
+    assert all(np.array_equal(x, y) for fa, fb in zip(got, want) for x, y in zip(fa, fb))
