@@ -292,10 +292,11 @@ __global__ void k_colseg(const uint8_t* __restrict__ dom, const uint8_t* __restr
     last[o] = l;
 }
 
-// Distance in its column from every pixel to the nearest zero (-1: none).
+// Distance in its column from every pixel to the nearest zero (-1: none),
+// and that zero's row (the upper one of two at the same distance).
 __global__ void k_colpass(const uint8_t* __restrict__ dom, const uint8_t* __restrict__ skel, int nx, int ny,
                           int nseg, const int* __restrict__ first, const int* __restrict__ last,
-                          int* __restrict__ g) {
+                          int* __restrict__ g, int* __restrict__ zrow) {
     const int z = blockIdx.y;
     const uint8_t* img = z ? skel : dom;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -318,28 +319,41 @@ __global__ void k_colpass(const uint8_t* __restrict__ dom, const uint8_t* __rest
             break;
         }
     for (int s = sg + 1; dn < 0 && s < nseg; s++) dn = F[(size_t)s * nx + j];
-    int d = -1;
-    if (up >= 0) d = i - up;
-    if (dn >= 0 && (d < 0 || dn - i < d)) d = dn - i;
-    g[(size_t)z * nx * ny + p] = d;
+    const int zr = up >= 0 && (dn < 0 || i - up <= dn - i) ? up : dn;
+    g[(size_t)z * nx * ny + p] = zr < 0 ? -1 : (zr <= i ? i - zr : zr - i);
+    zrow[(size_t)z * nx * ny + p] = zr;
 }
 
 // Squared distance to the nearest zero: min over the row of dx^2 + g^2
-// (-1: the image has no zero pixel).
-__global__ void k_rowpass(const int* __restrict__ g, int nx, int ny, int* __restrict__ k2) {
+// (-1: the image has no zero pixel), and for the skeleton image the index of
+// the first nearest zero in row-major order (ties resolved on (row, col):
+// within a column only the column's nearest zero can reach the minimum).
+__global__ void k_rowpass(const int* __restrict__ g, const int* __restrict__ zrow, int nx, int ny,
+                          int* __restrict__ k2, int* __restrict__ nearest) {
     const int z = blockIdx.y;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (int64_t)nx * ny) return;
     const int i = (int)(p / nx), j = (int)(p % nx);
     const int* row = g + (size_t)z * nx * ny + (size_t)i * nx;
+    const int* zr = zrow + (size_t)z * nx * ny + (size_t)i * nx;
     const int64_t NONE = INT64_MAX;
-    int64_t best = row[j] >= 0 ? (int64_t)row[j] * row[j] : NONE;
-    for (int r = 1; r < nx && (int64_t)r * r < best; r++) {
+    int64_t best = NONE, bq = INT64_MAX;
+    auto consider = [&](int k, int64_t dx2) {
+        if (row[k] < 0) return;
+        const int64_t v = dx2 + (int64_t)row[k] * row[k], q = (int64_t)zr[k] * nx + k;
+        if (v < best || (v == best && q < bq)) {
+            best = v;
+            bq = q;
+        }
+    };
+    consider(j, 0);
+    for (int r = 1; r < nx && (int64_t)r * r <= best; r++) {
         const int64_t r2 = (int64_t)r * r;
-        if (j - r >= 0 && row[j - r] >= 0) best = min(best, r2 + (int64_t)row[j - r] * row[j - r]);
-        if (j + r < nx && row[j + r] >= 0) best = min(best, r2 + (int64_t)row[j + r] * row[j + r]);
+        if (j - r >= 0) consider(j - r, r2);
+        if (j + r < nx) consider(j + r, r2);
     }
     k2[(size_t)z * nx * ny + p] = best == NONE ? -1 : (int)best;
+    if (z == 1) nearest[p] = best == NONE ? -1 : (int)bq;
 }
 
 // OpenCV's float value of a squared distance: sqrtf of the exact integer;
@@ -368,8 +382,9 @@ __device__ __forceinline__ int64_t isqrt_ceil(int64_t t) {
 // by one): in a row only the columns with dx^2 in [sLo - dy^2, sHi - dy^2]
 // are tested, in ascending order, with the reference's own double test.
 __global__ void k_props(const uint8_t* __restrict__ dom, const uint8_t* __restrict__ skel,
-                        const int* __restrict__ kw, const int* __restrict__ ks, int nx, int ny, double resolution,
-                        double mfp, double* __restrict__ locpore, double* __restrict__ kn, int* __restrict__ err) {
+                        const int* __restrict__ kw, const int* __restrict__ ks, const int* __restrict__ nearest, int nx,
+                        int ny, double resolution, double mfp, double* __restrict__ locpore, double* __restrict__ kn,
+                        int* __restrict__ err) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (int64_t)nx * ny) return;
     if (dom[p] != 1) {
@@ -385,6 +400,23 @@ __global__ void k_props(const uint8_t* __restrict__ dom, const uint8_t* __restri
         return;
     }
     const double D = (double)dist_value(ks[p]);
+    // Below 1024 the float value of D is within 6.1e-5 of sqrt(ks) and the
+    // next integer square is over 4.3e-4 away, so the reference's test passes
+    // exactly the skeleton pixels at squared distance ks: the first of them
+    // in row-major order is the distance pass's nearest.  It is checked with
+    // the reference's own test; the window search covers the rest.
+    if (D < 1024.0) {
+        const int q = nearest[p];
+        if (q >= 0) {
+            const double dy = (double)(i - q / nx), dx = (double)(j - q % nx);
+            if (fabs(__builtin_sqrt(dy * dy + dx * dx) - D) < 1.0e-4 && skel[q]) {
+                const double lp = 2.0 * (double)dist_value(kw[q]);
+                locpore[p] = lp;
+                kn[p] = mfp / lp / resolution;
+                return;
+            }
+        }
+    }
     const int64_t r0 = (int64_t)(i - D - 2), c0 = (int64_t)(j - D - 2);
     const double rEnd = i + D + 2, cEnd = j + D + 2;
     const double lo = D - 1.0e-4, hi = D + 1.0e-4;
@@ -437,7 +469,9 @@ struct mts_shape_s {
     int* first = nullptr;
     int* last = nullptr;
     int* g = nullptr;
+    int* zrow = nullptr;
     int* k2 = nullptr;
+    int* nearest = nullptr;
     unsigned* cnt = nullptr;
     uint8_t* chg = nullptr;  // two arrays of per-tile change flags
     int tilesX = 0, tilesY = 0;
@@ -489,6 +523,8 @@ int mts_shape_create(int nx, int ny, mts_shape* out) {
     if (rc == MTS_OK) rc = h->alloc(&h->last, 2 * (size_t)h->nseg * nx);
     if (rc == MTS_OK) rc = h->alloc(&h->g, 2 * np);
     if (rc == MTS_OK) rc = h->alloc(&h->k2, 2 * np);
+    if (rc == MTS_OK) rc = h->alloc(&h->zrow, 2 * np);
+    if (rc == MTS_OK) rc = h->alloc(&h->nearest, np);
     if (rc == MTS_OK) rc = h->alloc(&h->cnt, CHECK_EVERY);
     if (rc == MTS_OK) rc = h->alloc(&h->chg, 2 * (size_t)h->tilesX * h->tilesY);
     if (rc == MTS_OK) rc = h->alloc(&h->err, 1);
@@ -556,10 +592,11 @@ int mts_shape_run(mts_shape h, const uint8_t* domain, double resolution, double 
     k_crop<<<grid_of(np, 256), 256, 0, s>>>(h->cv[cur], h->WW, h->m, nx, ny, sk);
     k_colseg<<<dim3(grid_of((int64_t)h->nseg * nx, 256), 2), 256, 0, s>>>(domain, sk, nx, ny, h->nseg, h->first,
                                                                          h->last);
-    k_colpass<<<dim3(grid_of(np, 256), 2), 256, 0, s>>>(domain, sk, nx, ny, h->nseg, h->first, h->last, h->g);
-    k_rowpass<<<dim3(grid_of(np, 256), 2), 256, 0, s>>>(h->g, nx, ny, h->k2);
-    k_props<<<grid_of(np, 256), 256, 0, s>>>(domain, sk, h->k2, h->k2 + np, nx, ny, resolution, mfp, locpore, kn,
-                                            h->err);
+    k_colpass<<<dim3(grid_of(np, 256), 2), 256, 0, s>>>(domain, sk, nx, ny, h->nseg, h->first, h->last, h->g,
+                                                        h->zrow);
+    k_rowpass<<<dim3(grid_of(np, 256), 2), 256, 0, s>>>(h->g, h->zrow, nx, ny, h->k2, h->nearest);
+    k_props<<<grid_of(np, 256), 256, 0, s>>>(domain, sk, h->k2, h->k2 + np, h->nearest, nx, ny, resolution, mfp,
+                                            locpore, kn, h->err);
     SH_TRY(hipGetLastError());
     SH_TRY(hipMemcpyAsync(&herr, h->err, sizeof(int), hipMemcpyDeviceToHost, s));
     SH_TRY(hipStreamSynchronize(s));
