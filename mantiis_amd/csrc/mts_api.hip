@@ -267,16 +267,17 @@ static int upload_cpar(mts_sim_s* s, const std::vector<CellParam>& params, const
 // Device copy of a cell-order image (the host image must outlive the copies).
 static int upload_compact(mts_sim_s* s, const CompactImage& h, const float* cpar, CompactDev& d) {
     uint32_t* cw;
-    int32_t *rr, *aux, *cidx, *segs, *cpos;
+    uint32_t* rw;
+    int32_t *aux, *cidx, *segs, *cpos;
     uint16_t* pidx;
     uint8_t* outm;
     int rc;
-    if ((rc = s->alloc(&cw, h.ncell)) || (rc = s->alloc(&rr, 2 * h.ncell)) ||
+    if ((rc = s->alloc(&cw, h.ncell)) || (rc = s->alloc(&rw, h.ncell)) ||
         (rc = s->alloc(&pidx, h.ncell)) || (rc = s->alloc(&outm, h.ncell)) ||
         (rc = s->alloc(&aux, std::max<size_t>(h.aux.size(), 1))) || (rc = s->alloc(&cidx, h.cidx.size())) ||
         (rc = s->alloc(&segs, h.segs.size())) || (rc = s->alloc(&cpos, h.ncell)))
         return rc;
-    if ((rc = upload(cw, h.cw.data(), h.ncell * 4, s->stream)) || (rc = upload(rr, h.rr.data(), h.ncell * 8, s->stream)) ||
+    if ((rc = upload(cw, h.cw.data(), h.ncell * 4, s->stream)) || (rc = upload(rw, h.rw.data(), h.ncell * 4, s->stream)) ||
         (rc = upload(pidx, h.pidx.data(), h.ncell * 2, s->stream)) ||
         (rc = upload(outm, h.outm.data(), h.ncell, s->stream)) ||
         (rc = upload(aux, h.aux.data(), h.aux.size() * 4, s->stream)) ||
@@ -285,7 +286,7 @@ static int upload_compact(mts_sim_s* s, const CompactImage& h, const float* cpar
         (rc = upload(cpos, h.cellPos.data(), h.ncell * 4, s->stream)))
         return rc;
     d.cw = cw;
-    d.rr = rr;
+    d.rw = rw;
     d.pidx = pidx;
     d.outm = outm;
     d.aux = aux;

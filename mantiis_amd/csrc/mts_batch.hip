@@ -289,7 +289,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
         b->ncell[g] = n;
         b->cellPos[g] = ci.cellPos;
         C.cw.resize(base + n);
-        C.rr.resize(2 * (base + n));
+        C.rw.resize(base + n);
         C.pidx.resize(base + n);
         C.outm.resize(base + n);
         for (int64_t c = 0; c < n; c++) {
@@ -297,13 +297,9 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
             C.cw[base + c] = cw;
             C.pidx[base + c] = (uint16_t)(ci.pidx[c] + poff);
             C.outm[base + c] = ci.outm[c];
-            if (cw & CW_COMPLEX) {
-                C.rr[2 * (base + c)] = (int32_t)(ci.rr[2 * c] + naux0);
-                C.rr[2 * (base + c) + 1] = 0;
-            } else {
-                C.rr[2 * (base + c)] = (int32_t)(ci.rr[2 * c] + base);
-                C.rr[2 * (base + c) + 1] = (int32_t)(ci.rr[2 * c + 1] + base);
-            }
+            // rank offsets are relative to the cell; aux records move with the
+            // lattice's first record
+            C.rw[base + c] = (cw & CW_COMPLEX) ? (uint32_t)(ci.rw[c] + naux0) : ci.rw[c];
         }
         for (int64_t k = 0; k < ci.naux; k++) {
             const int32_t* r = &ci.aux[k * AUXW];
@@ -317,7 +313,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
         base += ((n + 255) / 256) * 256;
         // padding cells between lattices are never in a block
         C.cw.resize(base, 0);
-        C.rr.resize(2 * base, 0);
+        C.rw.resize(base, 0u);
         C.pidx.resize(base, 0);
         C.outm.resize(base, 0);
         if (g == 0) canonical_spec(C.spec);
@@ -334,7 +330,8 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     if (e != hipSuccess) return bail(set_error(MTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e)));
     int rc;
     uint32_t* cw;
-    int32_t *rr, *aux;
+    uint32_t* rw;
+    int32_t* aux;
     uint16_t* pidx;
     uint8_t* outm;
     float* cpar;
@@ -349,7 +346,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
         o[4] = (float)(q.r_srbb - (double)o[3]);
     }
     const size_t nA = (size_t)base * 9;
-    if ((rc = b->alloc(&cw, base)) || (rc = b->alloc(&rr, 2 * base)) || (rc = b->alloc(&pidx, base)) ||
+    if ((rc = b->alloc(&cw, base)) || (rc = b->alloc(&rw, base)) || (rc = b->alloc(&pidx, base)) ||
         (rc = b->alloc(&outm, base)) || (rc = b->alloc(&aux, std::max<size_t>(C.aux.size(), 1))) ||
         (rc = b->alloc(&cpar, b->cpar_h.size())) || (rc = b->alloc(&b->A[0], nA)) || (rc = b->alloc(&b->A[1], nA)) ||
         (rc = b->alloc(&b->mac, (size_t)base * 3)) || (rc = b->alloc(&b->acc, 2 * (size_t)nb)) ||
@@ -358,7 +355,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
         (rc = b->alloc(&b->d_nactive, 1)))
         return bail(rc);
     e = hipMemcpyAsync(cw, C.cw.data(), base * 4, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(rr, C.rr.data(), base * 8, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rw, C.rw.data(), base * 4, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(pidx, C.pidx.data(), base * 2, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(outm, C.outm.data(), base, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess && !C.aux.empty())
@@ -382,7 +379,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     upload_spec(C.spec, b->stream);
     upload_icsr(b->stream);
     b->dev.cw = cw;
-    b->dev.rr = rr;
+    b->dev.rw = rw;
     b->dev.pidx = pidx;
     b->dev.outm = outm;
     b->dev.aux = aux;
@@ -391,7 +388,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     b->dev.cstride = base;
     // drop the host copies of what now lives on the device
     C.cw = std::vector<uint32_t>();
-    C.rr = std::vector<int32_t>();
+    C.rw = std::vector<uint32_t>();
     C.pidx = std::vector<uint16_t>();
     C.outm = std::vector<uint8_t>();
     C.aux = std::vector<int32_t>();

@@ -87,15 +87,17 @@ int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, con
 //   cw (u32): bits 0..7 IN, 8..15 WALL (as in the link word), 16..21 TYPE,
 //             bit 22 COMPLEX, bit 23 (x, y-1) is a cell, bit 24 (x, y+1) is a
 //             cell, bits 25..30 x mod 64;
-//   ru, rd:   (interleaved in rr) the number of cells before the site (x, y+1) /
-//             (x, y-1) (y wraps),
-//             so that the sources of slots 4, 8, 7 are ru, ru-1, ru+[(x,y+1)
-//             is a cell] and those of slots 2, 5, 6 are rd, rd-1, rd+[...];
+//   rw (u32): ru - c in bits 0..15 and c - rd in bits 16..31 (signed 16-bit),
+//             where ru, rd are the numbers of cells before the site (x, y+1) /
+//             (x, y-1) (y wraps), so that the sources of slots 4, 8, 7 are ru,
+//             ru-1, ru+[(x,y+1) is a cell] and those of slots 2, 5, 6 are rd,
+//             rd-1, rd+[...]; a cell whose offsets do not fit (the wrapped
+//             rows of a torus) is COMPLEX;
 //             the x neighbours of a cell are the cells c-1 and c+1;
 //   outm:     the OUT bits (Shan-Chen neighbour sum).
 // A COMPLEX cell (lattice columns 0 and nx-1, x-seam sites, the generalized-
 // periodic cells and cells whose wall links share a specular slot) keeps its
-// sources, position and full link word in an aux record; its ru is the record.
+// sources, position and full link word in an aux record; its rw is the record.
 constexpr uint32_t CW_WALL_SHIFT = 8, CW_TYPE_SHIFT = 16;
 constexpr uint32_t CW_COMPLEX = 1u << 22, CW_ISC_D = 1u << 23, CW_ISC_U = 1u << 24;
 constexpr uint32_t CW_X_SHIFT = 25;
@@ -106,7 +108,7 @@ struct CompactImage {
     int64_t ncell = 0, cstride = 0, naux = 0;
     int ntx = 0;                   // tile columns of CTILE_W sites
     std::vector<uint32_t> cw;      // [ncell]
-    std::vector<int32_t> rr;       // [ncell][2]: ru, rd (COMPLEX: aux record, 0)
+    std::vector<uint32_t> rw;      // [ncell]: packed rank offsets (COMPLEX: aux record)
     std::vector<uint16_t> pidx;    // [ncell]
     std::vector<uint8_t> outm;     // [ncell]
     std::vector<int32_t> aux;      // [naux * AUXW]

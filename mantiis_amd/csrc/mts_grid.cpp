@@ -305,7 +305,7 @@ void build_compact(const LatticeView& g, CompactImage& ci) {
             ci.segs[(size_t)y * (ci.ntx + 1) + tx] = rank[(int64_t)y * nx + x];
         }
     ci.cw.assign(n, 0);
-    ci.rr.assign(2 * n, 0);
+    ci.rw.assign(n, 0u);
     ci.pidx.assign(n, 0);
     ci.outm.assign(n, 0);
     ci.aux.clear();
@@ -357,12 +357,15 @@ void build_compact(const LatticeView& g, CompactImage& ci) {
         };
         ci.pidx[c] = g.pidx[p];
         ci.outm[c] = (uint8_t)out;
-        if (complex) {
+        // rank offsets of the rows above and below, packed as two signed
+        // 16-bit values (the wrapped rows of a torus do not fit)
+        const int64_t du = (int64_t)rank[pu] - c, dd = c - (int64_t)rank[pd];
+        const bool fits = du >= -32768 && du <= 32767 && dd >= -32768 && dd <= 32767;
+        if (complex || !fits) {
             cw |= CW_COMPLEX;
-            ci.rr[2 * c] = push_aux();
+            ci.rw[c] = (uint32_t)push_aux();
         } else {
-            ci.rr[2 * c] = rank[pu];
-            ci.rr[2 * c + 1] = rank[pd];
+            ci.rw[c] = ((uint32_t)(uint16_t)(int16_t)du) | ((uint32_t)(uint16_t)(int16_t)dd << 16);
             // every streamed slot must find its source by rank arithmetic
             const int64_t ru = rank[pu], rd = rank[pd];
             const int64_t src[9] = {c, c - 1, rd, c + 1, ru, rd - 1, rd + ((cw & CW_ISC_D) ? 1 : 0),
@@ -375,8 +378,7 @@ void build_compact(const LatticeView& g, CompactImage& ci) {
             }
             if (!ok) {
                 cw |= CW_COMPLEX;
-                ci.rr[2 * c] = push_aux();
-                ci.rr[2 * c + 1] = 0;
+                ci.rw[c] = (uint32_t)push_aux();
             }
         }
         ci.cw[c] = cw;

@@ -11,7 +11,7 @@ namespace mts {
 // Device copy of a CompactImage (cell-order FP32 passes).
 struct CompactDev {
     const uint32_t* cw = nullptr;
-    const int32_t* rr = nullptr;  // interleaved (ru, rd)
+    const uint32_t* rw = nullptr;  // packed rank offsets / aux record
     const uint16_t* pidx = nullptr;
     const uint8_t* outm = nullptr;
     const int32_t* aux = nullptr;
