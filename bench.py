@@ -117,7 +117,7 @@ def run_to_steady(lat, tol, chunk, max_steps, max_s):
     return n, time.perf_counter() - t0, crit, lat.residual()
 
 
-PROFILE_FOR = {"c2": "k_cmb2_quadtree_transport_1024.md", "c3": "k_csc_shanchen_adsorption_2048.md",
+PROFILE_FOR = {"c2": "k_cmbx_quadtree_transport_1024.md", "c3": "k_csc_shanchen_adsorption_2048.md",
                "transport": "k_ctr_transport_4096.md"}
 
 
@@ -496,7 +496,8 @@ def main():
         workload = (f"quadtree multi-block transport RMRT+SRBB, {lat.nx}x{lat.ny} disk packing porosity {phi:.3f} "
                     f"(U[0.4,0.8]), Kn=0.5, torus, FP32 deviation; {n1} level-1 and {n2} level-2 cells "
                     "(2 level-1 substeps per coarse step)")
-        kernel = "k_cmb2 (first substep of both levels, RMRT+SRBB, stored-field write)"
+        kernel = ("k_cmbx (interface of the previous coarse step + first substep of both levels, RMRT+SRBB, "
+                  "stored-field write)")
         dtype = "fp32 (populations)"
     elif args.workload == "transport":
         from mantiis_amd.hip import Lattice
@@ -528,13 +529,13 @@ def main():
     step_achieved = bytes_per_cell * updates * args.steps / (ms / 1e3) / 1e9
     dominant = None
     if args.workload == "c2":
-        # The quadtree step is four short launches (~20 us each), for which
+        # The quadtree step is two short launches (~16 and ~32 us), for which
         # per-launch events add several us; the roofline is taken over the
         # whole coarse step from the events around the K-step call, and the
         # dominant launch is reported beside it.
         dominant = {"kernel": kernel, "event_ms": round(t_kernel_ms, 6), "cells_per_launch": kernel_cells,
                     "achieved": round(achieved, 1)}
-        kernel = "coarse step: k_cmb2 + k_cmb + k_if32 (coarse rows) + k_if32 (fine rows)"
+        kernel = "coarse step: k_cmbx (interface + first substeps) + k_cmb (second level-1 substep)"
         t_kernel_ms = ms / args.steps
         kernel_cells = updates
         achieved = step_achieved

@@ -9,6 +9,7 @@
 #include <fstream>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "mantiis_hip.h"
@@ -90,16 +91,22 @@ struct mts_sim_s {
     std::vector<CellParam> mparams;
     CoarseRow* d_crows = nullptr;
     FineRow* d_frows = nullptr;
-    CoarseRow32* d_crows32 = nullptr;
-    FineRow32* d_frows32 = nullptr;
     int64_t ncrows = 0, nfrows = 0;
+    // FP32 interface plan (one launch per coarse step)
+    IfHead* d_ifhead = nullptr;
+    IfTask* d_iftask = nullptr;
+    IfCoarse* d_ifco = nullptr;
+    IfFine* d_iffo = nullptr;
+    int nifb = 0;
     // FP32 multi-block state in cell order, per level: image (host and device,
     // with and without interface overrides), carried states, stored fields
+    // (two sets: a step's interface reads the previous step's while the
+    // step's first substeps write their own)
     struct LevelC {
         CompactImage h;
         CompactDev plain, ovr;
         float* A[2] = {nullptr, nullptr};
-        float* m = nullptr;
+        float* m[2] = {nullptr, nullptr};
         float* O = nullptr;
     };
     std::vector<LevelC> lc;
@@ -366,6 +373,192 @@ static void interface_rows(const MultiGeometry& G, const std::vector<LevelImage>
     }
 }
 
+// Interface plan of the FP32 path (IfHead / IfTask / IfCoarse / IfFine):
+// fine rows are packed in order into blocks while the block's gathers,
+// coarse-buffer values and fine outputs fit; a node that is a coarse-buffer
+// cell adds that coarse row (its children's gathers) to the block, and the
+// first block to add a coarse row writes its override.  The coarse rows no
+// fine row uses fill further blocks.  Every array is padded to the block's
+// capacity, so a block's records start at fixed offsets.
+struct IfPlan {
+    std::vector<IfHead> head;
+    std::vector<IfTask> task;
+    std::vector<IfCoarse> co;
+    std::vector<IfFine> fo;
+};
+
+static void build_if_plan(const std::vector<CoarseRow32>& cr, const std::vector<FineRow32>& fr, const CompactImage& h1,
+                          const CompactImage& h2, IfPlan& P) {
+    std::unordered_map<int32_t, int32_t> crowOf;  // coarse-buffer cell -> coarse row
+    for (size_t r = 0; r < cr.size(); r++)
+        if (cr[r].nch > 0) crowOf[cr[r].parent] = (int32_t)r;
+    std::vector<char> owned(cr.size(), 0);
+    std::unordered_map<int64_t, int> tslot;  // (level, cell) -> task slot of the open block
+    std::unordered_map<int32_t, int> cslot;  // coarse row -> value slot of the open block
+    std::vector<IfTask> T;
+    std::vector<IfCoarse> C;
+    std::vector<IfFine> F;
+    auto key = [](int lvl, int32_t cell) { return ((int64_t)lvl << 32) | (uint32_t)cell; };
+    auto flush = [&]() {
+        if (T.empty() && C.empty() && F.empty()) return;
+        // gathers in (level, cell) order, so that neighbouring lanes read
+        // neighbouring elements of each plane
+        std::vector<int> ord(T.size()), at(T.size());
+        for (size_t i = 0; i < T.size(); i++) ord[i] = (int)i;
+        std::sort(ord.begin(), ord.end(), [&](int x, int y) {
+            return std::make_pair(T[x].pl >> 31, T[x].cell) < std::make_pair(T[y].pl >> 31, T[y].cell);
+        });
+        std::vector<IfTask> Ts(T.size());
+        for (size_t i = 0; i < T.size(); i++) {
+            Ts[i] = T[ord[i]];
+            at[ord[i]] = (int)i;
+        }
+        T.swap(Ts);
+        for (IfCoarse& o : C)
+            for (int k = 0; k < 4; k++)
+                if (o.src[k] >= 0) o.src[k] = (int16_t)at[o.src[k]];
+        for (IfFine& o : F)
+            for (int k = 0; k < 6; k++)
+                if (o.node[k] < IF_TASKS) o.node[k] = (int16_t)at[o.node[k]];
+        P.head.push_back(IfHead{(int32_t)T.size(), (int32_t)C.size(), (int32_t)F.size(), 0});
+        T.resize(IF_TASKS, IfTask{-1, 0u, 0u, 0u});
+        C.resize(IF_CO, IfCoarse{{-1, -1, -1, -1}, 0, 0.0f, 0, -1, 0, 0});
+        F.resize(IF_FO, IfFine{0, 0, 0.0f, {0, 0, 0, 0, 0, 0}, 0, 0, {0, 0, 0, 0, 0, 0}});
+        P.task.insert(P.task.end(), T.begin(), T.end());
+        P.co.insert(P.co.end(), C.begin(), C.end());
+        P.fo.insert(P.fo.end(), F.begin(), F.end());
+        T.clear();
+        C.clear();
+        F.clear();
+        tslot.clear();
+        cslot.clear();
+    };
+    auto task = [&](int lvl, int32_t cell) -> int {
+        const auto it = tslot.find(key(lvl, cell));
+        if (it != tslot.end()) return it->second;
+        const CompactImage& h = lvl == 2 ? h2 : h1;
+        T.push_back(IfTask{cell, h.cw[cell], h.rw[cell], (uint32_t)h.pidx[cell] | (lvl == 2 ? 0x80000000u : 0u)});
+        return tslot[key(lvl, cell)] = (int)T.size() - 1;
+    };
+    auto coarse = [&](int32_t r) -> int {
+        const auto it = cslot.find(r);
+        if (it != cslot.end()) return IF_TASKS + it->second;
+        const CoarseRow32& R = cr[r];
+        IfCoarse o{{-1, -1, -1, -1}, R.parent, R.k, R.nch, owned[r] ? -1 : R.slot, (int32_t)h2.pidx[R.parent], 0};
+        owned[r] = 1;
+        for (int k = 0; k < 4; k++)
+            if (R.child[k] >= 0) o.src[k] = (int16_t)task(1, R.child[k]);
+        C.push_back(o);
+        return IF_TASKS + (cslot[r] = (int)C.size() - 1);
+    };
+    // gathers and coarse values a fine row would add to the open block
+    auto need = [&](const FineRow32& R, int& nt, int& nc) {
+        nt = nc = 0;
+        std::vector<int64_t> seen;
+        for (int k = 0; k < 6; k++) {
+            const auto it = crowOf.find(R.src[k]);
+            if (it == crowOf.end()) {
+                const int64_t kk = key(2, R.src[k]);
+                if (!tslot.count(kk) && std::find(seen.begin(), seen.end(), kk) == seen.end()) {
+                    seen.push_back(kk);
+                    nt++;
+                }
+            } else if (!cslot.count(it->second) &&
+                       std::find(seen.begin(), seen.end(), -1 - (int64_t)it->second) == seen.end()) {
+                seen.push_back(-1 - (int64_t)it->second);
+                nc++;
+                nt += cr[it->second].nch;
+            }
+        }
+    };
+    for (size_t r = 0; r < fr.size(); r++) {
+        const FineRow32& R = fr[r];
+        int nt, nc;
+        need(R, nt, nc);
+        if ((int)T.size() + nt > IF_TASKS || (int)C.size() + nc > IF_CO || (int)F.size() + 4 > IF_FO) flush();
+        int16_t node[6];
+        for (int k = 0; k < 6; k++) {
+            const auto it = crowOf.find(R.src[k]);
+            node[k] = (int16_t)(it == crowOf.end() ? task(2, R.src[k]) : coarse(it->second));
+        }
+        for (int ch = 0; ch < 4; ch++) {
+            IfFine o{R.child[ch], (int32_t)(4 * r + ch), R.ratio[ch], {0, 0, 0, 0, 0, 0}, h1.pidx[R.child[ch]], 0,
+                     {0, 0, 0, 0, 0, 0}};
+            for (int k = 0; k < 6; k++) {
+                o.node[k] = node[k];
+                o.L[k] = R.L[ch][k];
+            }
+            F.push_back(o);
+        }
+    }
+    flush();
+    for (size_t r = 0; r < cr.size(); r++) {
+        if (cr[r].nch <= 0 || owned[r]) continue;
+        if ((int)T.size() + cr[r].nch > IF_TASKS || (int)C.size() + 1 > IF_CO) flush();
+        coarse((int32_t)r);
+    }
+    flush();
+}
+
+// Bounds of every index the interface kernel takes from the plan, and each
+// override slot written exactly once; false names the first violation.
+static bool check_if_plan(const IfPlan& P, size_t ncr, const std::vector<CoarseRow32>& cr, size_t nfr, int64_t ncell1,
+                          int64_t ncell2, size_t nparam, std::string& err) {
+    std::vector<int> wrote2(ncr, 0), wrote1(4 * nfr, 0);
+    for (size_t b = 0; b < P.head.size(); b++) {
+        const IfHead& H = P.head[b];
+        if (H.ntask < 0 || H.ntask > IF_TASKS || H.nco < 0 || H.nco > IF_CO || H.nfo < 0 || H.nfo > IF_FO) {
+            err = "interface plan: block capacity";
+            return false;
+        }
+        for (int t = 0; t < H.ntask; t++) {
+            const IfTask& T = P.task[b * IF_TASKS + t];
+            if (T.cell < 0 || T.cell >= ((T.pl >> 31) ? ncell2 : ncell1)) {
+                err = "interface plan: task cell";
+                return false;
+            }
+        }
+        for (int i = 0; i < H.nco; i++) {
+            const IfCoarse& C = P.co[b * IF_CO + i];
+            for (int k = 0; k < 4; k++)
+                if (C.src[k] < -1 || C.src[k] >= H.ntask) {
+                    err = "interface plan: child slot";
+                    return false;
+                }
+            if (C.parent < 0 || C.parent >= ncell2 || C.nch <= 0 || C.out < -1 || C.out >= (int64_t)ncr ||
+                C.pidx < 0 || (size_t)C.pidx >= nparam) {
+                err = "interface plan: coarse record";
+                return false;
+            }
+            if (C.out >= 0) wrote2[C.out]++;
+        }
+        for (int i = 0; i < H.nfo; i++) {
+            const IfFine& F = P.fo[b * IF_FO + i];
+            for (int k = 0; k < 6; k++)
+                if (F.node[k] < 0 || (F.node[k] >= H.ntask && (F.node[k] < IF_TASKS || F.node[k] >= IF_TASKS + H.nco))) {
+                    err = "interface plan: node slot";
+                    return false;
+                }
+            if (F.child < 0 || F.child >= ncell1 || F.out < 0 || F.out >= (int64_t)(4 * nfr) || F.pidx >= nparam) {
+                err = "interface plan: fine record";
+                return false;
+            }
+            wrote1[F.out]++;
+        }
+    }
+    for (size_t r = 0; r < ncr; r++)
+        if (wrote2[r] != (cr[r].nch > 0 ? 1 : 0)) {
+            err = "interface plan: coarse row not written once";
+            return false;
+        }
+    for (size_t i = 0; i < wrote1.size(); i++)
+        if (wrote1[i] != 1) {
+            err = "interface plan: fine buffer not written once";
+            return false;
+        }
+    return true;
+}
+
 // Cell-order images of the levels for the FP32 path.  Buffer cells written
 // by the interface get an override slot: coarse buffer of row r -> r, fine
 // buffer k of row r -> 4 r + k.
@@ -427,13 +620,14 @@ static int create_multi32(mts_sim_s* s, const std::vector<CoarseRow>& cr, const 
         if ((rc = upload_compact(s, C.h, cpar, C.plain))) return rc;
         const int64_t OS = std::max<int64_t>((nslot + 63) / 64 * 64, 64);
         if ((rc = s->alloc(&C.A[0], (size_t)C.h.cstride * 9)) || (rc = s->alloc(&C.A[1], (size_t)C.h.cstride * 9)) ||
-            (rc = s->alloc(&C.m, (size_t)C.h.cstride * 3)) || (rc = s->alloc(&C.O, (size_t)OS * 9)))
+            (rc = s->alloc(&C.m[0], (size_t)C.h.cstride * 3)) || (rc = s->alloc(&C.m[1], (size_t)C.h.cstride * 3)) ||
+            (rc = s->alloc(&C.O, (size_t)OS * 9)))
             return rc;
         C.plain.O = C.O;
         C.plain.OS = OS;
-        C.plain.ovr = false;
+        C.plain.ovr = 0;
         C.ovr = C.plain;
-        C.ovr.ovr = true;
+        C.ovr.ovr = 1;
     }
     upload_spec(s->lc[0].h.spec, s->stream);
     // interface rows in cell indices with host-formed weights
@@ -470,12 +664,22 @@ static int create_multi32(mts_sim_s* s, const std::vector<CoarseRow>& cr, const 
                 o.ratio[ch] = (float)R.ratio[ch];
             }
         }
-        if ((rc = s->alloc((char**)&s->d_crows32, c32.size() * sizeof(CoarseRow32))) ||
-            (rc = s->alloc((char**)&s->d_frows32, f32.size() * sizeof(FineRow32))) ||
-            (rc = upload(s->d_crows32, c32.data(), c32.size() * sizeof(CoarseRow32), s->stream)) ||
-            (rc = upload(s->d_frows32, f32.data(), f32.size() * sizeof(FineRow32), s->stream)))
+        IfPlan P;
+        build_if_plan(c32, f32, s->lc[0].h, s->lc[1].h, P);
+        if (!check_if_plan(P, c32.size(), c32, f32.size(), s->lc[0].h.ncell, s->lc[1].h.ncell, s->mparams.size(), err))
+            return fail(MTS_ERR_GEOMETRY, err);
+        s->nifb = (int)P.head.size();
+        if (s->nifb > 0 &&
+            ((rc = s->alloc((char**)&s->d_ifhead, P.head.size() * sizeof(IfHead))) ||
+             (rc = s->alloc((char**)&s->d_iftask, P.task.size() * sizeof(IfTask))) ||
+             (rc = s->alloc((char**)&s->d_ifco, P.co.size() * sizeof(IfCoarse))) ||
+             (rc = s->alloc((char**)&s->d_iffo, P.fo.size() * sizeof(IfFine))) ||
+             (rc = upload(s->d_ifhead, P.head.data(), P.head.size() * sizeof(IfHead), s->stream)) ||
+             (rc = upload(s->d_iftask, P.task.data(), P.task.size() * sizeof(IfTask), s->stream)) ||
+             (rc = upload(s->d_ifco, P.co.data(), P.co.size() * sizeof(IfCoarse), s->stream)) ||
+             (rc = upload(s->d_iffo, P.fo.data(), P.fo.size() * sizeof(IfFine), s->stream))))
             return rc;
-        HIP_TRY(hipStreamSynchronize(s->stream));  // the host rows go out of scope
+        HIP_TRY(hipStreamSynchronize(s->stream));  // the host plan goes out of scope
     }
     return MTS_OK;
 }
@@ -552,17 +756,38 @@ static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const
 // (mantiisApp.hpp:239-246 with evolutionStepOfMultiBlock, coreLB.hpp:1310-1331).
 // FP32 quadtree step in cell order: per coarse step the coarse level does one
 // fused substep, the fine level two (the second with the stored fields of
-// the first), then the interface writes the buffer overrides.  A call
-// starts from and ends in the canonical FP64 level images.
+// the first), then the interface forms the buffer cells' post-stream states.
+// From the second step of a call on, the interface of the previous step runs
+// in the launch of the step's first substeps and updates the buffer cells
+// itself; the last step's interface writes the overrides.  A call starts
+// from and ends in the canonical FP64 level images.
 static int multi_transport32(mts_sim_s* s, int nsteps) {
     const int Lmax = (int)s->lv.size();
     int rc;
     int cur[2] = {0, 0};
+    int mi = 0;  // stored-field set of the current step
+    auto iface = [&](int mprev) {
+        IfSpec f;
+        f.A1 = s->lc[0].A[cur[0]];
+        f.A2 = s->lc[1].A[cur[1]];
+        f.m1 = s->lc[0].m[mprev];
+        f.m2 = s->lc[1].m[mprev];
+        f.O1 = s->lc[0].O;
+        f.O2 = s->lc[1].O;
+        f.head = s->d_ifhead;
+        f.task = s->d_iftask;
+        f.co = s->d_ifco;
+        f.fo = s->d_iffo;
+        f.nblocks = s->nifb;
+        return f;
+    };
     for (int step = 0; step < nsteps; step++) {
         const bool first = step == 0, last = step == nsteps - 1;
         if (Lmax == 2) {
-            // both levels' first substeps in one launch, then the second
+            // both levels' first substeps in one launch (with the previous
+            // step's interface after the first step), then the second
             // level-1 substep with the stored fields
+            const IfSpec f = iface(mi ^ 1);
             MbSpec a[2];
             LaunchCtx c[2];
             for (int l = 0; l < 2; l++) {
@@ -572,12 +797,13 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
                 a[l].Ain = C.A[cur[l]];
                 a[l].Aout = C.A[cur[l] ^ 1];
                 a[l].fcan = s->lv[l].d_f;
-                a[l].m = C.m;
+                a[l].m = C.m[mi];
                 a[l].usq_old = last ? s->lv[l].d_usq_old : nullptr;
                 cur[l] ^= 1;
             }
             s->tick(c[0]);
-            if ((rc = launch_cmultistep2(c[0], a[0], c[1], a[1], first ? 0 : 1)))
+            if ((rc = first ? launch_cmultistep2(c[0], a[0], c[1], a[1], 0)
+                            : launch_cinterface_step(c[0], a[0], c[1], a[1], f)))
                 return fail(rc, "operator combination has no kernel");
             s->tock(c[0]);
             auto& C = s->lc[0];
@@ -585,9 +811,12 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
             MbSpec b;
             b.Ain = C.A[cur[0]];
             b.Aout = C.A[cur[0] ^ 1];
-            b.m = C.m;
+            b.m = C.m[mi];
             launch_cmultistep(c[0], 1, 1, b);
             cur[0] ^= 1;
+            mi ^= 1;
+            if ((rc = s->launched())) return rc;
+            continue;
         }
         for (int l = Lmax == 2 ? -1 : Lmax - 1; l >= 0; l--) {
             auto& C = s->lc[l];
@@ -598,7 +827,7 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
             a.Ain = C.A[cur[l]];
             a.Aout = C.A[cur[l] ^ 1];
             a.fcan = L.d_f;
-            a.m = C.m;
+            a.m = C.m[0];
             a.usq_old = last ? L.d_usq_old : nullptr;
             if ((rc = launch_cmultistep(c, first ? 0 : 1, 0, a))) return fail(rc, "operator combination has no kernel");
             cur[l] ^= 1;
@@ -607,29 +836,18 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
                 MbSpec b;
                 b.Ain = C.A[cur[l]];
                 b.Aout = C.A[cur[l] ^ 1];
-                b.m = C.m;
+                b.m = C.m[0];
                 launch_cmultistep(c, 1, 1, b);
                 cur[l] ^= 1;
             }
         }
-        if (Lmax > 1) {
-            IfSpec f;
-            f.A1 = s->lc[0].A[cur[0]];
-            f.A2 = s->lc[1].A[cur[1]];
-            f.m1 = s->lc[0].m;
-            f.m2 = s->lc[1].m;
-            f.O1 = s->lc[0].O;
-            f.O2 = s->lc[1].O;
-            f.crows = s->d_crows32;
-            f.frows = s->d_frows32;
-            f.ncr = s->ncrows;
-            f.nfr = s->nfrows;
-            LaunchCtx c1 = s->lctx(0), c2 = s->lctx(1);
-            c1.cimg = &s->lc[0].ovr;
-            c2.cimg = &s->lc[1].ovr;
-            if ((rc = launch_cinterface(c1, c2, f))) return fail(rc, "interface has no kernel");
-        }
         if ((rc = s->launched())) return rc;
+    }
+    if (Lmax == 2) {  // the last step's interface, as overrides for the export
+        LaunchCtx c1 = s->lctx(0), c2 = s->lctx(1);
+        c1.cimg = &s->lc[0].ovr;
+        c2.cimg = &s->lc[1].ovr;
+        if ((rc = launch_cinterface(c1, c2, iface(mi ^ 1)))) return fail(rc, "interface has no kernel");
     }
     // export: post-stream state with the interface overrides -> canonical
     for (int l = 0; l < Lmax; l++) {

@@ -23,7 +23,7 @@ struct CompactDev {
     int ntx = 0;
     const float* O = nullptr;  // interface overrides (multi-block buffer cells)
     int64_t OS = 0;
-    bool ovr = false;
+    int ovr = 0;  // 0 gather, 1 take the override, 2 skip (see CImg)
 };
 
 struct LaunchCtx {
@@ -86,15 +86,54 @@ struct FineRow32 {
     float L[4][6];     // Lagrange weights of the 3 x 2 node grid per child
     float ratio[4];
 };
-// Interface of the multi-block FP32 path.
+// Interface of the multi-block FP32 path: a block gathers up to IF_TASKS
+// post-stream states into LDS, forms up to IF_CO coarse-buffer values from
+// them and up to IF_FO fine-buffer values.  A fine row's interpolation node
+// that is a coarse-buffer cell takes the value formed in the same block from
+// its children, so no block waits for another.  The values are either
+// written as overrides, or are the buffer cells' post-stream states of the
+// next step, updated in the same launch as that step's other cells.
+constexpr int IF_TASKS = 256, IF_CO = 64, IF_FO = 256;
+struct IfHead {
+    int32_t ntask, nco, nfo, pad;
+};
+struct IfTask {    // one post-stream state to gather
+    int32_t cell;  // cell of its level image
+    uint32_t cw, rw;
+    uint32_t pl;   // parameter index, bit 31: level 2
+};
+struct IfCoarse {     // a coarse-buffer value (coreLB.hpp:1101-1150)
+    int16_t src[4];   // task slots of the children (-1 absent)
+    int32_t parent;   // level-2 cell (stored fields)
+    float k;          // 2 tau_c / mean tau_f
+    int32_t nch;
+    int32_t out;      // override slot to write, -1: used in this block only
+    int32_t pidx;     // parameter index of the parent
+    int32_t pad;
+};
+struct IfFine {       // one fine-buffer value (coreLB.hpp:1152-1231)
+    int32_t child;    // level-1 cell (stored fields)
+    int32_t out;      // override slot 4 r + k
+    float ratio;
+    int16_t node[6];  // LDS value slots of the interpolation nodes
+    uint16_t pidx;    // parameter index of the child
+    int16_t pad;
+    float L[6];       // Lagrange weights of the 3 x 2 node grid
+};
 struct IfSpec {
     const float *A1 = nullptr, *A2 = nullptr;
     const float *m1 = nullptr, *m2 = nullptr;
     float *O1 = nullptr, *O2 = nullptr;
-    const CoarseRow32* crows = nullptr;
-    const FineRow32* frows = nullptr;
-    int64_t ncr = 0, nfr = 0;
+    const IfHead* head = nullptr;
+    const IfTask* task = nullptr;   // [nblocks * IF_TASKS]
+    const IfCoarse* co = nullptr;   // [nblocks * IF_CO]
+    const IfFine* fo = nullptr;     // [nblocks * IF_FO]
+    int nblocks = 0;
 };
+// Interface of one coarse step fused with the next step's first substeps
+// (launch_cmultistep2 with the overrides formed in the launch).
+int launch_cinterface_step(const LaunchCtx& c1, const MbSpec& s1, const LaunchCtx& c2, const MbSpec& s2,
+                           const IfSpec& f);
 
 void upload_icsr(hipStream_t st);
 // FP32 production kernels (mts_fast.inc)
