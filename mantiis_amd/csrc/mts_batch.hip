@@ -64,11 +64,13 @@ struct mts_batch_s {
     bool tuned = false;
     std::vector<double> tune_ms;  // per candidate: ms per step (-1: not timed)
     float* snap = nullptr;        // carried state saved while tuning
-    // graph of one chunk of steps (replayed while the chunk length holds)
+    // graph of one chunk of steps (replayed while the chunk length and the
+    // grid hold)
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int gchunk = 0;
     int gcur = -1;
+    int ggrid = -1;
 
     template <class T>
     int alloc(T** p, size_t n) {
@@ -113,6 +115,17 @@ struct mts_batch_s {
         BT_TRY(hipMemcpyAsync(d_nactive, &nactive, 4, hipMemcpyHostToDevice, stream));
         return hipStreamSynchronize(stream) == hipSuccess ? MTS_OK : set_error(MTS_ERR_HIP, "queue upload");
     }
+    // Grid of a step: the active blocks rounded up to a power of two (at
+    // least 1024), so that a shrinking queue is not launched over every
+    // block of the batch (blocks past the active count exit at once, but
+    // hundreds of thousands of them still cost their dispatch), while the
+    // captured chunk is replaced only when the bucket changes.
+    int grid_for(int n) const {
+        if (n <= 0) return 0;
+        int g = 1024;
+        while (g < n) g *= 2;
+        return std::min<int>(g, (int)hblocks.size());
+    }
     // one step over the queue: A[cur] -> A[cur^1]
     int step(bool stats, int from) {
         BatchLaunch L;
@@ -122,7 +135,7 @@ struct mts_batch_s {
         L.blocks = d_blocks;
         L.queue = d_queue;
         L.nactive = d_nactive;
-        L.maxblocks = (int)hblocks.size();
+        L.maxblocks = grid_for(nactive);
         L.mac = stats ? mac : nullptr;
         L.part = stats ? part : nullptr;
         L.from_rest = from;
@@ -203,7 +216,7 @@ struct mts_batch_s {
         return MTS_OK;
     }
     int chunk_graph(int n) {
-        if (!gexec || gchunk != n || gcur != cur) {
+        if (!gexec || gchunk != n || gcur != cur || ggrid != grid_for(nactive)) {
             if (gexec) (void)hipGraphExecDestroy(gexec);
             if (graph) (void)hipGraphDestroy(graph);
             gexec = nullptr;
@@ -220,6 +233,7 @@ struct mts_batch_s {
             gchunk = n;
             cur = c0;  // capture did not run anything
             gcur = c0;
+            ggrid = grid_for(nactive);
         }
         BT_TRY(hipGraphLaunch(gexec, stream));
         if (n & 1) cur ^= 1;
