@@ -70,6 +70,30 @@ def c2_inputs(n=1024, seed=2025):
     return pore, kn, lp, phi
 
 
+def copy_peak_gbs(launches=10):
+    """Measured copy bandwidth of this GPU: the library's 16-byte-per-lane
+    nontemporal copy of 1 GiB (read + write bytes per second), timed with HIP
+    events on the stream it runs on.  The roofline's practical ceiling, beside
+    the nominal 8 TB/s."""
+    from mantiis_amd.hip import DeviceArray, load_library
+    from mantiis_amd.hiptime import StreamTimer
+    n = 1 << 28
+    a, b = DeviceArray((n,)), DeviceArray((n,))
+    lib = load_library()
+    if lib.mts_probe_copy16(b.ptr, a.ptr, n, 2) != 0:
+        raise RuntimeError(lib.mts_last_error().decode())
+    t = StreamTimer(0)
+    t.start()
+    rc = lib.mts_probe_copy16(b.ptr, a.ptr, n, launches)
+    ms = t.stop()
+    t.close()
+    a.close()
+    b.close()
+    if rc != 0:
+        raise RuntimeError(lib.mts_last_error().decode())
+    return 2 * 4 * n * launches / (ms / 1e3) / 1e9
+
+
 def mean_ux(lat):
     d = lat.diagnostics()  # sum rho, sum j_x, sum j_y, cells
     return d[1] / d[0]
@@ -91,6 +115,8 @@ def c4_one(args):
 
 
 def c4_inputs(nb=256, seed=4000, workers=8):
+    if workers <= 1:  # no forked workers (e.g. under a profiler that follows forks)
+        return [c4_one((seed + i,)) for i in range(nb)]
     import multiprocessing as mp
     with mp.get_context("fork").Pool(workers) as pool:
         return pool.map(c4_one, [(seed + i,) for i in range(nb)])
@@ -225,7 +251,7 @@ def bench_c4(args):
     from mantiis_amd.hip import Batch
     from mantiis_amd.hiptime import StreamTimer
     t0 = time.perf_counter()
-    geoms = c4_inputs(args.batch)
+    geoms = c4_inputs(args.batch, workers=args.workers)
     t_gen = time.perf_counter() - t0
     t0 = time.perf_counter()
     b = Batch([(g[0], g[1], g[2]) for g in geoms], collision=2, wall=2, dx=1e-9, Fbody=1e-6)
@@ -233,7 +259,7 @@ def bench_c4(args):
     cells = b.cells()
     t = StreamTimer(b.stream_handle())
     t.start()
-    r = b.run(400000, check_every=100, tol=1e-6, graph=True)
+    r = b.run(400000, check_every=100, tol=1e-6, graph=not args.eager)
     ms = t.stop()
     t.close()
     thr, xrun, tab = b.tune(-1)  # the configuration the run's autotuner chose
@@ -242,8 +268,11 @@ def bench_c4(args):
     conv = r["steps"] > 0
     sizes = np.array([g[0].shape[0] for g in geoms])
     by_size = {int(L): {"lattices": int((sizes == L).sum()),
-                        "median_steps": int(np.median(np.abs(r["steps"][sizes == L]))) if (sizes == L).any() else 0}
+                        "median_steps": int(np.median(np.abs(r["steps"][sizes == L]))) if (sizes == L).any() else 0,
+                        "max_steps": int(np.abs(r["steps"][sizes == L]).max()) if (sizes == L).any() else 0}
                for L in (128, 256, 512, 1024)}
+    b.close()
+    copy = copy_peak_gbs()
     out = {
         "metric": "MLUPS & %HBM roofline",
         "value": round(mlups, 2),
@@ -265,9 +294,11 @@ def bench_c4(args):
         "roofline": {"bound": "hbm", "kernel": "k_cbat (one launch per step over the queue of unconverged lattices; "
                                                "whole run incl. convergence checks)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_cell_update": 72},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_cell_update": 72,
+                     "copy_peak": round(copy, 1), "frac_of_copy": round(achieved / copy, 4)},
         "cpu_baseline": None,
         "batch": {"converged": int(conv.sum()), "run_s": round(ms / 1e3, 3), "cell_updates": int(r["updates"]),
+                  "graph": not args.eager,
                   "generation_s": round(t_gen, 1), "setup_s": round(t_setup, 1)},
         "autotune": {"threads_per_block": thr, "xcd_run": xrun,
                      "ms_per_step": [[round(v, 4) for v in row] for row in tab.tolist()],
@@ -433,6 +464,10 @@ def main():
     ap.add_argument("--eval", type=int, default=64, help="c5: unseen evaluation lattices (512^2)")
     ap.add_argument("--epochs", type=int, default=20, help="c5: training epochs")
     ap.add_argument("--batch", type=int, default=256, help="c4: number of lattices")
+    ap.add_argument("--workers", type=int, default=8, help="c4/c5: processes generating the geometries")
+    ap.add_argument("--eager", action="store_true",
+                    help="c4: launch the steps one by one instead of replaying a captured graph per chunk "
+                         "(rocprofv3's kernel trace fails inside HIP graph instantiation)")
     ap.add_argument("--no-steady", action="store_true", help="c2: skip the run to steady state")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU sample steps (default: c2 1000, c3 16)")
@@ -579,6 +614,9 @@ def main():
         "cpu_baseline": cpu,
         "mass": float(diag[0]),
     }
+    copy = copy_peak_gbs()
+    out["roofline"]["copy_peak"] = round(copy, 1)
+    out["roofline"]["frac_of_copy"] = round(achieved / copy, 4)
     if args.workload == "c2":
         out["config"]["cell_updates_per_step"] = updates
         out["roofline"]["dominant_launch"] = dominant

@@ -330,6 +330,12 @@ int mts_dev_download(void* dst, const void* src, size_t bytes);
  * FETCH_SIZE / WRITE_SIZE counters. */
 int mts_probe_copy(float* dst, const float* src, int64_t n, int launches);
 
+/* Bandwidth copy (no reference counterpart): as mts_probe_copy with one
+ * 16-byte nontemporal load and store per lane; n a multiple of 4 and 16-byte
+ * aligned pointers (MTS_ERR_ARG otherwise).  Its rate is the measured copy
+ * ceiling beside the nominal HBM peak. */
+int mts_probe_copy16(float* dst, const float* src, int64_t n, int launches);
+
 /* ------------------------------------------------------------------------
  * Geometry preprocessing: replaces Shape::calculateProperties
  * (include/shape.hpp:127-251 of the reference) -- Zhang-Suen skeleton of the

@@ -1705,6 +1705,17 @@ int mts_probe_copy(float* dst, const float* src, int64_t n, int launches) {
     return MTS_OK;
 }
 
+int mts_probe_copy16(float* dst, const float* src, int64_t n, int launches) {
+    if (!dst || !src || n <= 0 || n % 4 || n > ((int64_t)1 << 38) || launches <= 0 || ((uintptr_t)dst & 15) ||
+        ((uintptr_t)src & 15))
+        return fail(MTS_ERR_ARG, "bad argument (n a multiple of 4, 16-byte aligned pointers)");
+    for (int i = 0; i < launches; i++) launch_probe_copy16(dst, src, n, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("probe copy: ") + hipGetErrorString(e));
+    return MTS_OK;
+}
+
 int mts_kernel_timer(mts_sim s, int enable) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
     s->fold_timer();

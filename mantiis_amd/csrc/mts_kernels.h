@@ -181,6 +181,7 @@ void launch_gather(const LaunchCtx& c, const double* dense, int q, double* compa
 void launch_scatter(const LaunchCtx& c, double* dense, int q, const double* compact);
 void launch_init(const LaunchCtx& c, double* fcan);
 void launch_probe_copy(float* dst, const float* src, int64_t n, hipStream_t st);
+void launch_probe_copy16(float* dst, const float* src, int64_t n, hipStream_t st);
 void launch_macro(const LaunchCtx& c, const double* fcan, double* rho, double* ux, double* uy, double* usq);
 // Multi-block interface (coreLB.hpp:1101-1231) on the canonical FP64 arrays.
 struct CoarseRow {  // coarse buffer site (level 2) and its 4 children (level-1 sites, -1 absent)

@@ -32,7 +32,7 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_step_coupled", "mts_residual", "mts_method", "mts_field_size", "mts_get_field",
            "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_get_stream", "mts_synchronize",
            "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows",
-           "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy", "mts_batch_create", "mts_batch_destroy",
+           "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy", "mts_probe_copy16", "mts_batch_create", "mts_batch_destroy",
            "mts_batch_cells", "mts_batch_run", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
            "mts_batch_stream", "mts_batch_tune", "mts_batch_set_config", "mts_dev_alloc", "mts_dev_free", "mts_dev_upload", "mts_dev_download",
            "mts_net_create", "mts_net_destroy", "mts_net_param_count", "mts_net_get_params", "mts_net_set_params",
@@ -106,6 +106,7 @@ def load_library(path=LIB_PATH):
     lib.mts_kernel_timer.argtypes = [C.c_void_p, C.c_int]
     lib.mts_kernel_timer_read.argtypes = [C.c_void_p, _D, _I64]
     lib.mts_probe_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
+    lib.mts_probe_copy16.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
     _PP = C.POINTER(C.c_void_p)
     lib.mts_batch_create.argtypes = [C.c_int, _I32, _I32, _PP, _PP, _PP, C.POINTER(Params), _PP]
     lib.mts_batch_destroy.argtypes = [C.c_void_p]
