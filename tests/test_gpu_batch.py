@@ -118,3 +118,35 @@ def test_batch_launch_configurations_agree_and_tuning_is_invisible():
     tb.run(200, check_every=100, tol=1e-300)
     got = [tb.fields(g) for g in range(len(gs))]
     assert all(np.array_equal(x, y) for fa, fb in zip(got, want) for x, y in zip(fa, fb))
+
+
+def test_batch_grid_follows_the_queue():
+    """The step grid is the active block count rounded up to a power of two
+    (at least 1024); the captured chunk is replaced when that bucket changes.
+    Four 512^2 lattices (~2400 blocks of 256 cells) are deactivated one by one,
+    which crosses the 4096 / 2048 / 1024 buckets; every stage's graph replays
+    must equal eager launches bit for bit, and inactive lattices must not
+    change."""
+    banner("batch grid buckets: 4 x 512^2, lattices deactivated one by one, graph vs eager")
+    gs = []
+    for phi, kn, seed in ((0.6, 0.1, 11), (0.65, 0.05, 12), (0.55, 0.3, 13), (0.7, 0.02, 14)):
+        pore = G.disk_packing(512, phi, 3, 12, seed=seed)
+        k, lp = G.uniform_kn(pore, kn, MFP_DX)
+        gs.append((pore, k, lp))
+    a = Batch(gs, **PARAMS)
+    b = Batch(gs, **PARAMS)
+    blocks = sum((int(n) + 255) // 256 for n in a.cells())
+    assert blocks > 2048
+    for stage in range(4):
+        mask = np.array([g >= stage for g in range(4)])
+        a.activate(mask)
+        b.activate(mask)
+        frozen = [a.fields(g) for g in range(stage)]
+        ra = a.run(40, check_every=40, tol=1e-300, graph=True)
+        rb = b.run(40, check_every=40, tol=1e-300, graph=False)
+        assert ra["updates"] == rb["updates"] == 40 * sum(int(n) for n in a.cells()[stage:])
+        for g in range(4):
+            fa, fb = a.fields(g), b.fields(g)
+            assert all(np.array_equal(x, y) for x, y in zip(fa, fb)), (stage, g)
+        for g in range(stage):
+            assert all(np.array_equal(x, y) for x, y in zip(frozen[g], a.fields(g))), (stage, g)
