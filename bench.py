@@ -1,8 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X D2Q9 step (one JSON line on stdout).
 
-Default workload (--workload c2): the quadtree multi-block transport
-configuration -- a 1024x1024 seeded disk packing with porosity drawn from
+Default workload (--workload quadtree): the quadtree multi-block transport
+case -- a 1024x1024 seeded disk packing with porosity drawn from
 U[0.4, 0.8], Kn = 0.5 at the finest level, regularised MRT collision with
 specular bounce-back, periodic in x and y, FP32 deviation storage.  A "step"
 is one coarse step of the quadtree: one fused substep of the coarse level,
@@ -10,20 +10,25 @@ two of the fine level, and the interface update.  The run then continues
 from rest to steady state (mean-velocity change below 1e-6 per step) and
 reports the wall time.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|transport|c1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload quadtree|shanchen|transport|fp64|batched|warmstart|shape] [--no-cpu]
 
 value = cell updates per second (MLUPS; a coarse step counts 2 N_1 + N_2) over
 exactly K steps timed with HIP events on the library's stream, inputs
-resident in HBM.  roofline: the step's dominant kernel (the first substep of
-both levels, one launch), its average launch time from HIP events recorded
-around every launch inside the timed region, and 72 B per cell update (nine
-FP32 populations read and written; metadata, stored fields and interface
-traffic not counted).  The CPU baseline is the oracle (the C++ restatement
-of the reference, -Ofast -fopenmp) timed on a bounded sample.
+resident in HBM.  roofline: 72 B per cell update (nine FP32 populations read
+and written; metadata, stored fields and interface traffic not counted) over
+the coarse step's time, with the dominant launch reported beside it; for the
+single-block workloads, the fused pass's average launch time from HIP events
+recorded around it.  copy_peak: the measured copy ceiling of the same GPU.
+The CPU baseline is the oracle (the C++ restatement of the reference,
+-Ofast -fopenmp) timed on a bounded sample.
 
-Other workloads: c3 (Shan-Chen adsorption, 2048^2, 88 B per update),
-transport (single-block FP32 transport of a C2-distribution lattice, --n),
-c1 (single-block FP64 transport, 256^2, 144 B per update).
+Other workloads: shanchen (Shan-Chen adsorption, 2048^2, 88 B per update),
+transport (single-block FP32 transport of a lattice of the quadtree
+workload's distribution, --n), fp64 (single-block FP64 transport, 256^2,
+144 B per update), batched (256 lattices of uneven size and Kn to steady
+state in one batch), warmstart (the learned warm start against plain runs),
+shape (Shape::calculateProperties, 1024^2).
 """
 import argparse
 import json
@@ -39,7 +44,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def c3_inputs(n=2048, seed=2027):
+def shanchen_inputs(n=2048, seed=2027):
     from mantiis_amd import geometry as G
     rng = np.random.default_rng(seed)
     pore = G.disk_packing(n, 0.5, 4, 12, seed=seed)
@@ -51,7 +56,7 @@ def c3_inputs(n=2048, seed=2027):
 
 
 def transport_inputs(n=1024, seed=2026):
-    """C2-distribution geometry (porosity U[0.4, 0.8], Kn = 0.5) run single-block."""
+    """Quadtree-workload geometry (porosity U[0.4, 0.8], Kn = 0.5) run single-block."""
     from mantiis_amd import geometry as G
     rng = np.random.default_rng(seed)
     phi = float(rng.uniform(0.4, 0.8))
@@ -60,7 +65,7 @@ def transport_inputs(n=1024, seed=2026):
     return pore, kn, lp
 
 
-def c2_inputs(n=1024, seed=2025):
+def quadtree_inputs(n=1024, seed=2025):
     """Quadtree configuration: seeded disk packing, porosity U[0.4, 0.8], Kn = 0.5."""
     from mantiis_amd import geometry as G
     rng = np.random.default_rng(seed)
@@ -99,7 +104,7 @@ def mean_ux(lat):
     return d[1] / d[0]
 
 
-def c4_one(args):
+def batched_one(args):
     """One lattice of the batched configuration: side L in {128, 256, 512,
     1024}, porosity U[0.4, 0.8], disk radius U[3, 16], Kn log-uniform in
     [0.01, 1] (LocPore = (mfp/dx)/Kn)."""
@@ -114,12 +119,12 @@ def c4_one(args):
     return pore, knf, lp, kn
 
 
-def c4_inputs(nb=256, seed=4000, workers=8):
+def batched_inputs(nb=256, seed=4000, workers=8):
     if workers <= 1:  # no forked workers (e.g. under a profiler that follows forks)
-        return [c4_one((seed + i,)) for i in range(nb)]
+        return [batched_one((seed + i,)) for i in range(nb)]
     import multiprocessing as mp
     with mp.get_context("fork").Pool(workers) as pool:
-        return pool.map(c4_one, [(seed + i,) for i in range(nb)])
+        return pool.map(batched_one, [(seed + i,) for i in range(nb)])
 
 
 def run_to_steady(lat, tol, chunk, max_steps, max_s):
@@ -143,7 +148,7 @@ def run_to_steady(lat, tol, chunk, max_steps, max_s):
     return n, time.perf_counter() - t0, crit, lat.residual()
 
 
-PROFILE_FOR = {"c2": "k_cmbx_quadtree_transport_1024.md", "c3": "k_csc_shanchen_adsorption_2048.md",
+PROFILE_FOR = {"quadtree": "k_cmbx_quadtree_transport_1024.md", "shanchen": "k_csc_shanchen_adsorption_2048.md",
                "transport": "k_ctr_transport_4096.md"}
 
 
@@ -167,9 +172,9 @@ def w9():
     return np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
 
 
-def make_c3(precision="fp32dev"):
+def make_shanchen(precision="fp32dev"):
     from mantiis_amd.hip import Lattice
-    pore, kn, lp, G_ads, rho0, rho = c3_inputs()
+    pore, kn, lp, G_ads, rho0, rho = shanchen_inputs()
     lat = Lattice(pore, kn, lp, precision=precision, collision=1, wall=0, dx=1e-9, Fbody=0.0, Gfs=G_ads, periodic_y=1)
     fl = pore.reshape(-1) == 1
     r = rho.reshape(-1)[fl]
@@ -200,13 +205,13 @@ def time_steps(lat, kind, steps, warmup):
     return ms, kms, klaunch
 
 
-def cpu_baseline_c2(sample_n, sample_steps):
+def cpu_baseline_quadtree(sample_n, sample_steps):
     """Oracle multi-block transport on a sample_n^2 lattice of the same
     distribution (the reference's quadtree build is quadratic in the
     lattice size, so the 1024^2 lattice itself takes minutes to set up)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
-    pore, kn, lp, phi = c2_inputs(sample_n)
+    pore, kn, lp, phi = quadtree_inputs(sample_n)
     o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-6, fast=True, periodic_y=True, multiblock=True)
     lvl = o.grid()["level"]
     upd = int(2 * (lvl == 1).sum() + (lvl == 2).sum()) if lvl.max() > 1 else int(len(lvl))
@@ -244,14 +249,14 @@ def cpu_baseline(inputs, sample_steps):
                        f"serial as in the reference)")
 
 
-def bench_c4(args):
+def bench_batched(args):
     """The batched configuration: every lattice from rest to steady state
     (mean-velocity change below 1e-6 per step, checked every 100 steps) in
     one batch; value = cell updates per second over that run."""
     from mantiis_amd.hip import Batch
     from mantiis_amd.hiptime import StreamTimer
     t0 = time.perf_counter()
-    geoms = c4_inputs(args.batch, workers=args.workers)
+    geoms = batched_inputs(args.batch, workers=args.workers)
     t_gen = time.perf_counter() - t0
     t0 = time.perf_counter()
     b = Batch([(g[0], g[1], g[2]) for g in geoms], collision=2, wall=2, dx=1e-9, Fbody=1e-6)
@@ -309,7 +314,7 @@ def bench_c4(args):
     return 0
 
 
-def bench_c5(args):
+def bench_warmstart(args):
     """Learned warm start: train the encoder-decoder on steady states of
     --train 256^2 lattices of the batched distribution (from the batched
     solver), then solve --eval unseen 512^2 lattices from rest and from the
@@ -459,19 +464,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "transport", "c1", "shape"])
-    ap.add_argument("--train", type=int, default=2000, help="c5: training lattices (256^2)")
-    ap.add_argument("--eval", type=int, default=64, help="c5: unseen evaluation lattices (512^2)")
-    ap.add_argument("--epochs", type=int, default=20, help="c5: training epochs")
-    ap.add_argument("--batch", type=int, default=256, help="c4: number of lattices")
-    ap.add_argument("--workers", type=int, default=8, help="c4/c5: processes generating the geometries")
+    ap.add_argument("--workload", default="quadtree", choices=["quadtree", "shanchen", "batched", "warmstart", "transport", "fp64", "shape"])
+    ap.add_argument("--train", type=int, default=2000, help="warmstart: training lattices (256^2)")
+    ap.add_argument("--eval", type=int, default=64, help="warmstart: unseen evaluation lattices (512^2)")
+    ap.add_argument("--epochs", type=int, default=20, help="warmstart: training epochs")
+    ap.add_argument("--batch", type=int, default=256, help="batched: number of lattices")
+    ap.add_argument("--workers", type=int, default=8, help="batched/warmstart: processes generating the geometries")
     ap.add_argument("--eager", action="store_true",
-                    help="c4: launch the steps one by one instead of replaying a captured graph per chunk "
+                    help="batched: launch the steps one by one instead of replaying a captured graph per chunk "
                          "(rocprofv3's kernel trace fails inside HIP graph instantiation)")
-    ap.add_argument("--no-steady", action="store_true", help="c2: skip the run to steady state")
+    ap.add_argument("--no-steady", action="store_true", help="quadtree: skip the run to steady state")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=0, help="CPU sample steps (default: c2 1000, c3 16)")
-    ap.add_argument("--cpu-n", type=int, default=256, help="c2: lattice side of the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=0, help="CPU sample steps (default: quadtree 1000, shanchen 16)")
+    ap.add_argument("--cpu-n", type=int, default=256, help="quadtree: lattice side of the CPU sample")
     ap.add_argument("--n", type=int, default=0, help="lattice side of the transport workload (default 1024)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy (torch) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -501,14 +506,14 @@ def main():
             return 1
         print(json.dumps({"copy_probe_bytes": n * 4, "launches": 5}))
         return 0
-    if args.workload == "c4":
-        return bench_c4(args)
-    if args.workload == "c5":
-        return bench_c5(args)
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "warmstart":
+        return bench_warmstart(args)
     if args.workload == "shape":
         return bench_shape(args)
-    if args.workload == "c3":
-        lat, inputs = make_c3()
+    if args.workload == "shanchen":
+        lat, inputs = make_shanchen()
         kind = "adsorption"
         bytes_per_cell = 88
         workload = ("Shan-Chen adsorption, 2048x2048 seeded disk packing porosity 0.5, "
@@ -516,9 +521,9 @@ def main():
                     "MRT + Guo source + bounce-back, FP32 deviation storage, FP64 psi")
         kernel = "k_csc (tiled Shan-Chen: forces, MRT + Guo source, bounce-back, generalized periodic)"
         dtype = "fp32 (populations) / fp64 (psi, moments)"
-    elif args.workload == "c2":
+    elif args.workload == "quadtree":
         from mantiis_amd.hip import Lattice
-        pore, kn, lp, phi = c2_inputs(args.n or 1024)
+        pore, kn, lp, phi = quadtree_inputs(args.n or 1024)
         lat = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
                       multiblock=1)
         lvl = lat.grid()["level"]
@@ -555,7 +560,7 @@ def main():
         dtype = "fp64"
 
     n = lat.ncell
-    if args.workload != "c2":
+    if args.workload != "quadtree":
         updates = kernel_cells = n
     ms, kms, klaunch = time_steps(lat, kind, args.steps, args.warmup)
     mlups = updates * args.steps / (ms / 1e3) / 1e6
@@ -563,7 +568,7 @@ def main():
     achieved = bytes_per_cell * kernel_cells / (t_kernel_ms / 1e3) / 1e9
     step_achieved = bytes_per_cell * updates * args.steps / (ms / 1e3) / 1e9
     dominant = None
-    if args.workload == "c2":
+    if args.workload == "quadtree":
         # The quadtree step is two short launches (~16 and ~32 us), for which
         # per-launch events add several us; the roofline is taken over the
         # whole coarse step from the events around the K-step call, and the
@@ -575,7 +580,7 @@ def main():
         kernel_cells = updates
         achieved = step_achieved
     steady = None
-    if args.workload == "c2" and not args.no_steady:
+    if args.workload == "quadtree" and not args.no_steady:
         from mantiis_amd.hip import Lattice
         lat2 = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
                        multiblock=1)
@@ -586,8 +591,8 @@ def main():
     diag = lat.diagnostics()
 
     cpu = None
-    if not args.no_cpu and args.workload == "c2":
-        cpu = cpu_baseline_c2(args.cpu_n, args.cpu_steps or 1000)
+    if not args.no_cpu and args.workload == "quadtree":
+        cpu = cpu_baseline_quadtree(args.cpu_n, args.cpu_steps or 1000)
     elif not args.no_cpu and inputs is not None:
         cpu = cpu_baseline(inputs, args.cpu_steps or 16)
 
@@ -617,7 +622,7 @@ def main():
     copy = copy_peak_gbs()
     out["roofline"]["copy_peak"] = round(copy, 1)
     out["roofline"]["frac_of_copy"] = round(achieved / copy, 4)
-    if args.workload == "c2":
+    if args.workload == "quadtree":
         out["config"]["cell_updates_per_step"] = updates
         out["roofline"]["dominant_launch"] = dominant
     if steady is not None:
