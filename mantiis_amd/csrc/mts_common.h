@@ -86,7 +86,7 @@ int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, con
 // in row-major order) and run one thread per cell.  Per cell:
 //   cw (u32): bits 0..7 IN, 8..15 WALL (as in the link word), 16..21 TYPE,
 //             bit 22 COMPLEX, bit 23 (x, y-1) is a cell, bit 24 (x, y+1) is a
-//             cell, bits 25..30 x mod 64;
+//             cell, bits 25..31 x mod 128;
 //   rw (u32): ru - c in bits 0..15 and c - rd in bits 16..31 (signed 16-bit),
 //             where ru, rd are the numbers of cells before the site (x, y+1) /
 //             (x, y-1) (y wraps), so that the sources of slots 4, 8, 7 are ru,
@@ -101,7 +101,7 @@ int build_single_grid(int nx, int ny, const int32_t* pore, const double* kn, con
 constexpr uint32_t CW_WALL_SHIFT = 8, CW_TYPE_SHIFT = 16;
 constexpr uint32_t CW_COMPLEX = 1u << 22, CW_ISC_D = 1u << 23, CW_ISC_U = 1u << 24;
 constexpr uint32_t CW_X_SHIFT = 25;
-constexpr int CTILE_W = 64, CTILE_H = 16;  // Shan-Chen tile (sites)
+constexpr int CTILE_W = 128, CTILE_H = 8;  // Shan-Chen tile (sites)
 constexpr int AUXW = 12;  // aux record: src[8], x, y, link word, override slot (-1: none)
 
 struct CompactImage {

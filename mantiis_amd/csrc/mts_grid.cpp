@@ -319,7 +319,7 @@ void build_compact(const LatticeView& g, CompactImage& ci) {
         const uint32_t t = (w >> TYPE_SHIFT) & 63u;
         const int yu = wrap(y + 1, ny), yd = wrap(y - 1, ny);
         const int64_t pu = (int64_t)yu * nx + x, pd = (int64_t)yd * nx + x;
-        uint32_t cw = in | (wall << CW_WALL_SHIFT) | (t << CW_TYPE_SHIFT) | ((uint32_t)(x & 63) << CW_X_SHIFT);
+        uint32_t cw = in | (wall << CW_WALL_SHIFT) | (t << CW_TYPE_SHIFT) | ((uint32_t)(x & 127) << CW_X_SHIFT);
         if (ci.cidx[pd] >= 0) cw |= CW_ISC_D;
         if (ci.cidx[pu] >= 0) cw |= CW_ISC_U;
         // specular source of each slot (SRBB, coreLB.hpp:918-933)
