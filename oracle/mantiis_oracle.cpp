@@ -10,17 +10,31 @@
 // Every function cites the reference file:line it restates.  Parity pins:
 //   * whole single-block transport / adsorption trajectories: the reference's
 //     own gold files test/test_input_output/gold/{transport,adsorption}
-//     (64x64, 100 steps) -- tests/test_oracle_gold.py;
+//     (64x64, 100 steps) -- tests/test_oracle_gold.py; they also pin the
+//     link tables (stream, wall, periodic) of that geometry, whose every
+//     wall pattern enters the trajectory;
 //   * constants (PR-EOS root, rho_in, mfp, Cf): the reference's printed
 //     values and the cubic's residual -- tests/test_oracle_physics.py;
 //   * equilibrium / regularisation / MRT / SRT: closed-form moment identities
 //     and MRT(all rates 1) == BGK(tau 1) -- tests/test_oracle_physics.py;
-//   * link tables: brute-force classification from first principles on tiny
-//     grids -- tests/test_oracle_links.py;
-//   * quadtree multi-block grid: brute-force invariants on tiny images
-//     (tests/test_oracle_multiblock.py).  The fine-buffer bicubic of the
-//     reference rests on ALGLIB (not vendored, version unpinned); the oracle's
-//     own bicubic is checked only by invariants: parity unpinned.
+//   * walls and seams: mass invariants of BB / SRBB / torus and the loss at a
+//     fluid y-seam under the reference's seam rule -- test_oracle_physics.py;
+//     the 52-type boundary tables against the neighbourhood rule --
+//     tests/test_tables.py;
+//   * quadtree multi-block grid: brute-force invariants on small images and
+//     an independent second build (the library's) bit for bit --
+//     tests/test_grid.py;
+//   * fine-buffer interpolation (ALGLIB spline2dbuildbicubicv on the 3 x 2
+//     nodes the reference passes, coreLB.hpp:1217-1224): a product of a
+//     quadratic in x and a linear function in y sampled at the parent's
+//     neighbours is reproduced exactly at every child's centre (a bicubic
+//     Hermite patch with spline-derived slopes reproduces such a product;
+//     a transposed Z order, swapped top/bottom rows or a wrong centre shift
+//     does not) -- tests/test_oracle_multiblock_pins.py;
+//   * per-level downsampling (cv::resize + threshold 128, utils.hpp:539-568):
+//     every 2 x 2 pattern against OpenCV's documented area (factor 2) and
+//     bilinear (factor 4) sampling with its 8-bit rounding --
+//     tests/test_oracle_multiblock_pins.py.
 // Build: oracle/Makefile (strict IEEE: -O2 -ffp-contract=off, the checker)
 // and an -Ofast -fopenmp copy used only as the timed CPU baseline.
 // ============================================================================
@@ -214,6 +228,7 @@ static std::vector<int> ind2sub(const std::vector<int>& size, int64_t index) {
 // ---------------------------------------------------------------- grid
 struct Grid {
     int nx = 0, ny = 0, maxLevel = 0;
+    int maxAllowedLevel = 2;                   // grid.hpp:1180 (a parameter here)
     bool isMultigrid = false;
     int64_t size = 0;                          // globalGridSize == localGridSize (np = 1)
     std::vector<int64_t> gridID;               // grid.hpp:167
@@ -352,7 +367,7 @@ static void getBoundaryTypes(Grid& G, const Tables& T) {
 struct Sim {
     Globals g;
     Tables T;
-    Shape S;
+    orc::Shape S;
     Grid G;
     int NX = 0, NY = 0;
     int t = 0;
@@ -862,7 +877,9 @@ extern "C" {
 const char* orc_last_error() { return g_err.c_str(); }
 
 // Inputs are row-major [y][x] (the in-memory Shape::domain layout after the
-// transposed load of shape.hpp:253-285).  multiblock=1 runs the quadtree path.
+// transposed load of shape.hpp:253-285).  multiblock=1 runs the quadtree path
+// with the reference's maxAllowedLevel = 2 (grid.hpp:1180); multiblock = k >= 2
+// runs it with maxAllowedLevel = k.
 void* orc_create(int N, const int* pore, const double* kn, const double* locpore, double dx,
                  double Fbody, double T, double P, const char* tables_dir, int multiblock, int periodic_y) {
     try {
@@ -877,6 +894,7 @@ void* orc_create(int N, const int* pore, const double* kn, const double* locpore
         s->S.Kn.assign(kn, kn + (size_t)N * N);
         s->S.LocPore.assign(locpore, locpore + (size_t)N * N);
         s->G.nx = N; s->G.ny = N; s->G.isMultigrid = multiblock != 0;
+        s->G.maxAllowedLevel = multiblock >= 2 ? multiblock : 2;
         std::vector<int64_t> img, linInd;
         if (multiblock) {
             orc::processMultigrid(s->S, s->G, img, s->periodicY);
@@ -1141,6 +1159,17 @@ void orc_distance(const uint8_t* fg, int rows, int cols, float* value, int64_t* 
 }
 int64_t orc_nearest_set(const uint8_t* img, int rows, int cols, int i, int j, double D) {
     return shp::nearestSetPixel(img, rows, cols, i, j, D);
+}
+// utils.hpp:539-568 at factor f (1, 2, 4, ...): the n/f x n/f image that
+// cv::resize + threshold(128) gives for the n x n 0/1 image dom (row-major);
+// the per-level relaxation time uses its interior rows.
+void orc_downsample(const int* dom, int n, int f, int* out) {
+    orc::Shape S;
+    S.Nx = S.Ny = n;
+    S.domain.assign(dom, dom + (size_t)n * n);
+    const int m = n / f;
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) out[(size_t)i * m + j] = orc::resizedPixel(S, f, i, j);
 }
 int orc_threads() {
 #ifdef _OPENMP

@@ -69,7 +69,18 @@ def load(fast=False):
     lib.orc_distance.argtypes = [_U8, C.c_int, C.c_int, C.POINTER(C.c_float), _L]
     lib.orc_nearest_set.restype = C.c_int64
     lib.orc_nearest_set.argtypes = [_U8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+    lib.orc_downsample.argtypes = [_I, C.c_int, C.c_int, _I]
     return lib
+
+
+def downsample(img, f):
+    """The oracle's per-level image (utils.hpp:539-568) of a square 0/1 image
+    at factor f, flattened row-major."""
+    img = np.ascontiguousarray(img, dtype=np.int32)
+    n = img.shape[0]
+    out = np.zeros((n // f) * (n // f), np.int32)
+    load().orc_downsample(_ptr(img, _I), n, f, _ptr(out, _I))
+    return out
 
 
 def mfp_of(dx=1e-9, T=300.0, P=2e6):
@@ -144,7 +155,7 @@ class Oracle:
         self._kn = np.ascontiguousarray(kn, dtype=np.float64)
         self._lp = np.ascontiguousarray(locpore, dtype=np.float64)
         h = self.lib.orc_create(self.n, _ptr(self._pore, _I), _ptr(self._kn, _D), _ptr(self._lp, _D),
-                                dx, Fbody, T, P, TABLES.encode(), 1 if multiblock else 0, 1 if periodic_y else 0)
+                                dx, Fbody, T, P, TABLES.encode(), int(multiblock), 1 if periodic_y else 0)
         if not h:
             raise RuntimeError(self.lib.orc_last_error().decode())
         self.h = h
@@ -233,6 +244,16 @@ class Oracle:
         c = np.zeros(n, np.int32)
         self.lib.orc_links(self.h, w, _ptr(a, _L), _ptr(b, _L), _ptr(c, _I))
         return a, b, c
+
+
+def oracle_rows_key(o, which, key):
+    """Buffer rows {parent, 4 children} of the oracle's coarseBuffer (which 0)
+    or fineBuffer (which 1) map at key (grid.hpp:1041-1108)."""
+    n = o.lib.orc_buffer(o.h, which, key, None)
+    out = np.zeros((n, 5), np.int64)
+    if n:
+        o.lib.orc_buffer(o.h, which, key, _ptr(out, _L))
+    return out
 
 
 def oracle_rows(o, which):

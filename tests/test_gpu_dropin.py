@@ -1,8 +1,9 @@
-"""Drivers written like the reference's test/transport_test.cpp and
-test/adsorption_test.cpp, compiled against this repository's include/ and
-run the way test/run_tests.sh runs them (1 thread, single-block, 64, 100
-steps); outputs compared with the gold files at the reference's criterion
-(max-abs 1e-10, scripts/compare_files.py:16-19)."""
+"""MantiisApp through this repository's include/ (tests/drivers/app_driver.cpp
+takes the Settings from its command line), run the way test/run_tests.sh runs
+the reference's regression cases (1 thread, single-block, 64, 100 steps, the
+Settings of test/transport_test.cpp:9-16 and test/adsorption_test.cpp:10-17);
+outputs compared with the gold files at the reference's criterion (max-abs
+1e-10, scripts/compare_files.py:16-19)."""
 import os
 import shutil
 import subprocess
@@ -33,12 +34,23 @@ def fixture_copy(tmp_path):
     return str(d) + "/"
 
 
+# Settings of the reference's regression drivers (test/transport_test.cpp:9-16,
+# test/adsorption_test.cpp:10-17)
+GOLD_SETTINGS = {
+    "transport": ["type=transport", "verbose=10", "T=300", "P=2e6", "dx=1e-9", "fbody=1e-8", "macro_every=1000000",
+                  "dist_first=-1"],
+    "adsorption": ["type=adsorption", "verbose=10", "T=300", "P=2e6", "dx=12e-9", "fbody=1e-8",
+                   "macro_every=1000000", "dist_first=-1"],
+}
+
+
 @pytest.mark.parametrize("kind", ["transport", "adsorption"])
-def test_reference_style_driver_reproduces_gold(kind, tmp_path):
-    banner(f"drop-in {kind} driver 64x64 100 steps")
-    exe = build(kind + "_driver", tmp_path)
+def test_app_reproduces_gold(kind, tmp_path):
+    banner(f"drop-in MantiisApp {kind} 64x64 100 steps")
+    exe = build("app_driver", tmp_path)
     folder = fixture_copy(tmp_path)
-    r = subprocess.run([str(exe), "1", "False", "64", folder, "100"], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([str(exe), *GOLD_SETTINGS[kind], "--", "1", "False", "64", folder, "100"],
+                       capture_output=True, text=True, timeout=120)
     print(r.stdout[-600:])
     assert r.returncode == 0, r.stderr
     for name in ("ux", "uy", "rho"):
@@ -91,7 +103,7 @@ def test_resume_from_dump_continues_exactly(tmp_path):
     the binary dump of f, equal the uninterrupted 150 steps bit for bit; from
     the 15-digit text dump they agree to the text's precision (absolute)."""
     banner("drop-in resume from f.bin / f.txt, fixture 64x64 transport")
-    exe = build("resume_driver", tmp_path)
+    exe = build("app_driver", tmp_path)
     runs = {}
     for name in ("a", "b", "c", "d"):
         d = tmp_path / name
@@ -100,12 +112,15 @@ def test_resume_from_dump_continues_exactly(tmp_path):
             shutil.copy(os.path.join(O.FIXTURES, f), d / f)
         runs[name] = str(d) + "/"
     env = dict(os.environ, OMP_NUM_THREADS="1")
-    run = lambda name, nt, *extra: subprocess.run([str(exe), "1", "false", "64", runs[name], str(nt), *extra],
-                                                  check=True, timeout=300, env=env, stdout=subprocess.DEVNULL)
+    base = ["type=transport", "verbose=1000", "dx=1e-9", "fbody=1e-8", "macro_every=1000000", "dist_first=-1",
+            "binary=1"]
+    run = lambda name, nt, *extra: subprocess.run([str(exe), *base, *extra, "--", "1", "false", "64", runs[name],
+                                                   str(nt)], check=True, timeout=300, env=env,
+                                                  stdout=subprocess.DEVNULL)
     run("a", 100)
-    run("b", 50, runs["a"] + "f.bin")
+    run("b", 50, "init=" + runs["a"] + "f.bin")
     run("c", 150)
-    run("d", 50, runs["a"] + "f.txt")
+    run("d", 50, "init=" + runs["a"] + "f.txt")
     for field in ("f", "ux", "uy", "rho"):
         b = np.fromfile(runs["b"] + field + ".bin")
         c = np.fromfile(runs["c"] + field + ".bin")
