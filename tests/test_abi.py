@@ -77,7 +77,7 @@ def test_load_existing_model_errors(tmp_path):
     assert e.value.code == -3 and "Insufficient data" in str(e.value)
 
 
-@pytest.mark.parametrize("driver", ["transport_driver", "adsorption_driver", "phase_driver"])
+@pytest.mark.parametrize("driver", ["app_driver", "phase_driver", "shape_driver"])
 def test_drivers_compile_and_link(driver, tmp_path):
     src = os.path.join(ROOT, "tests", "drivers", driver + ".cpp")
     exe = tmp_path / driver
@@ -87,6 +87,8 @@ def test_drivers_compile_and_link(driver, tmp_path):
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     # argc < 6 exits with the usage line (mantiisApp.hpp:131-135)
-    if driver != "phase_driver":
-        r = subprocess.run([str(exe)], capture_output=True, text=True)
+    if driver == "app_driver":
+        r = subprocess.run([str(exe), "type=transport", "--", "1"], capture_output=True, text=True)
         assert r.returncode == 1 and "Usage:" in r.stderr
+        r = subprocess.run([str(exe), "nonsense=1", "--"], capture_output=True, text=True)
+        assert r.returncode == 2 and "bad setting" in r.stderr
