@@ -263,6 +263,12 @@ int mts_batch_cells(mts_batch b, int64_t* cells);
  * use_graph captures a chunk in a hipGraph and replays it. */
 int mts_batch_run(mts_batch b, int64_t max_steps, int check_every, double tol, int use_graph, int64_t* steps,
                   double* seconds, double* ux_mean, int64_t* cell_updates);
+/* nsteps steps of the active lattices, one launch each, with no statistics
+ * and no convergence test (the fixed-length step of a timing window; it
+ * tunes first if the batch was never tuned).  Asynchronous on the batch's
+ * stream; mts_batch_fields reports the state of the last statistics step
+ * (creation or the last check of mts_batch_run). */
+int mts_batch_step(mts_batch b, int nsteps);
 /* marks lattices active (1) or not (0) for the next runs */
 int mts_batch_activate(mts_batch b, const int32_t* active);
 /* rho, u_x, u_y (nx*ny row-major, solids 0) of lattice g's post-stream

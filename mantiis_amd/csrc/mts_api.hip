@@ -30,6 +30,14 @@ static int fail(int code, const std::string& msg) {
     return code;
 }
 
+// Stores the residual of a fused step call (LB2D::diff) and reports a NaN or
+// Inf in it: a diverged run stops with MTS_ERR_NONFINITE instead of running
+// on (its test diff < tol is never true).
+static int set_residual(double* diff, double v) {
+    *diff = v;
+    return std::isfinite(v) ? MTS_OK : fail(MTS_ERR_NONFINITE, "residual is not finite (the run diverged)");
+}
+
 namespace mts {
 // error reporting shared with the batched solver (mts_batch.hip)
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
@@ -876,8 +884,7 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
         s1 += r[0];
         s2 += r[1];
     }
-    s->diff = s1 / s2;
-    return MTS_OK;
+    return set_residual(&s->diff, s1 / s2);
 }
 
 static int multi_transport(mts_sim_s* s, int nsteps) {
@@ -931,8 +938,7 @@ static int multi_transport(mts_sim_s* s, int nsteps) {
         s1 += r[0];
         s2 += r[1];
     }
-    s->diff = s1 / s2;
-    return MTS_OK;
+    return set_residual(&s->diff, s1 / s2);
 }
 
 // Downloads one per-cell field of a multi-block lattice into compact order.
@@ -1398,8 +1404,7 @@ int mts_step_transport(mts_sim s, int nsteps) {
     if ((rc = s->launched())) return rc;
     double r[4];
     if ((rc = s->reduce(0, s->d_usq, s->d_usq_old, nullptr, r))) return rc;
-    s->diff = r[0] / r[1];
-    return MTS_OK;
+    return set_residual(&s->diff, r[0] / r[1]);
 }
 
 static int sc_steps(mts_sim s, int kind, int nsteps) {
@@ -1448,7 +1453,7 @@ static int sc_steps(mts_sim s, int kind, int nsteps) {
         if ((rc = s->reduce(1, s->d_rho, s->d_rho_old, nullptr, r))) return rc;
         double d = r[0];
         if (d < 1.0e-15) d = 1.0e-5;  // coreLB.hpp:1280
-        s->diff = d;
+        s->diff = std::isfinite(r[1]) ? d : r[1];  // the max drops NaN, the sum does not
     } else {
         // coreLB.hpp:1248-1267 with usq_old never refreshed by this loop
         if ((rc = s->reduce(0, s->d_usq, s->d_usq_old, nullptr, r))) return rc;

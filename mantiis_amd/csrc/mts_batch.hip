@@ -180,11 +180,20 @@ struct mts_batch_s {
         }
         const int cur0 = cur;
         BT_TRY(hipMemcpyAsync(snap, A[cur0], elems * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        // puts the saved state back (also after a failed step: tuning never
+        // leaves the batch partly advanced)
+        auto restore = [&]() -> int {
+            cur = cur0;
+            if (hipMemcpyAsync(A[cur0], snap, elems * sizeof(float), hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+                hipStreamSynchronize(stream) != hipSuccess)
+                return set_error(MTS_ERR_HIP, "tune: restoring the state");
+            return MTS_OK;
+        };
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        BT_TRY(hipEventCreate(&e0));
-        BT_TRY(hipEventCreate(&e1));
         int best = -1;
         int rc = MTS_OK;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+            rc = set_error(MTS_ERR_HIP, "tune: event creation");
         for (int k = 0; k < NCAND && rc == MTS_OK; k++) {
             threads = cand_threads(k);
             xrun = cand_xrun(k);
@@ -195,19 +204,23 @@ struct mts_batch_s {
             if (rc) break;
             float ms = 0.0f;
             if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
                 rc = set_error(MTS_ERR_HIP, "tune: event timing");
+                break;
+            }
             tune_ms[k] = (double)ms / each;
             if (best < 0 || tune_ms[k] < tune_ms[best]) best = k;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (rc) return rc;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        const int rrc = restore();
+        if (rc || rrc || best < 0) {
+            threads = 256;
+            xrun = 32;
+            return rc ? rc : rrc;
+        }
         threads = cand_threads(best);
         xrun = cand_xrun(best);
-        BT_TRY(hipMemcpyAsync(A[cur0], snap, elems * sizeof(float), hipMemcpyDeviceToDevice, stream));
-        BT_TRY(hipStreamSynchronize(stream));
-        cur = cur0;
         // a captured chunk holds the old configuration
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -226,12 +239,19 @@ struct mts_batch_s {
             const int rc = chunk_eager(n);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(stream, &g);
-            if (rc) return rc;
-            if (e != hipSuccess) return set_error(MTS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            cur = c0;  // capture ran nothing, whatever chunk_eager flipped
+            if (rc || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc ? rc : set_error(MTS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            }
             graph = g;
-            BT_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                graph = nullptr;
+                gexec = nullptr;
+                return set_error(MTS_ERR_HIP, "graph instantiation");
+            }
             gchunk = n;
-            cur = c0;  // capture did not run anything
             gcur = c0;
             ggrid = grid_for(nactive);
         }
@@ -481,6 +501,19 @@ int mts_batch_run(mts_batch b, int64_t max_steps, int check_every, double tol, i
         if (ux_mean) ux_mean[g] = b->active[g] || st[g] > 0 ? um[g] : b->u_prev[g];
     }
     if (cell_updates) *cell_updates = upd;
+    return MTS_OK;
+}
+
+int mts_batch_step(mts_batch b, int nsteps) {
+    if (int rc = check(b)) return rc;
+    if (nsteps <= 0) return set_error(MTS_ERR_ARG, "nsteps must be positive");
+    if (!b->tuned) {
+        if (int rc = b->tune(8)) return rc;
+    }
+    for (int i = 0; i < nsteps; i++)
+        if (int rc = b->step(false, 0)) return rc;
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(MTS_ERR_HIP, std::string("step: ") + hipGetErrorString(e));
     return MTS_OK;
 }
 

@@ -136,9 +136,6 @@ int launch_cinterface_step(const LaunchCtx& c1, const MbSpec& s1, const LaunchCt
                            const IfSpec& f);
 
 void upload_icsr(hipStream_t st);
-// FP32 production kernels (mts_fast.inc)
-int launch_transport32(const LaunchCtx& c, int mode, const TransportSpec& s);
-int launch_shanchen32(const LaunchCtx& c, int kind, int mode, const ScSpec& s);
 // FP32 production kernels in cell order (mts_compact.inc)
 void upload_spec(const uint32_t* spec64, hipStream_t st);
 int launch_ctransport(const LaunchCtx& c, int mode, const TransportSpec& s);

@@ -556,7 +556,7 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // kind 0: {sum | |u| - |u_old| |, sum |u|} (coreLB.hpp:1248-1267)
-// kind 1: {max |rho - rho_old| / rho_old, 0}  (coreLB.hpp:1269-1283)
+// kind 1: {max |rho - rho_old| / rho_old, sum of the same}  (coreLB.hpp:1269-1283)
 // kind 2: {sum rho, sum rho*ux... } diagnostics: {sum rho, sum jx, sum jy, count}
 __global__ void __launch_bounds__(256) k_reduce(int kind, const uint32_t* meta, int64_t P, const double* a0,
                                                 const double* a1, const double* a2, double* part) {
@@ -571,6 +571,7 @@ __global__ void __launch_bounds__(256) k_reduce(int kind, const uint32_t* meta, 
         } else if (kind == 1) {
             const double d = fabs(a0[p] - a1[p]) / a1[p];
             s0 = fmax(s0, d);
+            s1 += d;  // fmax drops a NaN; the sum keeps it for the finiteness check
         } else {
             s0 += a0[p];
             s1 += a0[p] * a1[p];
@@ -579,8 +580,10 @@ __global__ void __launch_bounds__(256) k_reduce(int kind, const uint32_t* meta, 
         }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (kind == 1) s0 = wave_max(s0);
-    else {
+    if (kind == 1) {
+        s0 = wave_max(s0);
+        s1 = wave_sum(s1);
+    } else {
         s0 = wave_sum(s0);
         s1 = wave_sum(s1);
         s2 = wave_sum(s2);
@@ -745,8 +748,7 @@ int launch_transport(const LaunchCtx& c, int mode, const TransportSpec& s) {
     a.min_ux = s.min_ux;
     a.min_uy = s.min_uy;
     a.min_usq = s.min_usq;
-    if (c.fp32 && c.cimg) return launch_ctransport(c, mode, s);
-    if (c.fp32) return launch_transport32(c, mode, s);
+    if (c.fp32) return c.cimg ? launch_ctransport(c, mode, s) : MTS_ERR_STATE;  // FP32 runs in cell order
     if (mode == 0) return dispatch_t<double, 0>(c, a, c.coll, c.wall);
     if (mode == 1) return dispatch_t<double, 1>(c, a, c.coll, c.wall);
     return dispatch_t<double, 2>(c, a, c.coll, c.wall);
@@ -801,8 +803,7 @@ int launch_shanchen(const LaunchCtx& c, int kind, int mode, const ScSpec& s) {
     a.Fx = s.Fx;
     a.Fy = s.Fy;
     a.psi_negative = s.psi_negative;
-    if (c.fp32 && c.cimg) return launch_cshanchen(c, kind, mode, s);
-    if (c.fp32) return launch_shanchen32(c, kind, mode, s);
+    if (c.fp32) return c.cimg ? launch_cshanchen(c, kind, mode, s) : MTS_ERR_STATE;
     sc_dispatch<double>(c, kind, mode, a);
     return 0;
 }

@@ -33,7 +33,7 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_get_stream", "mts_synchronize",
            "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows",
            "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy", "mts_probe_copy16", "mts_batch_create", "mts_batch_destroy",
-           "mts_batch_cells", "mts_batch_run", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
+           "mts_batch_cells", "mts_batch_run", "mts_batch_step", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
            "mts_batch_stream", "mts_batch_tune", "mts_batch_set_config", "mts_dev_alloc", "mts_dev_free", "mts_dev_upload", "mts_dev_download",
            "mts_net_create", "mts_net_destroy", "mts_net_param_count", "mts_net_get_params", "mts_net_set_params",
            "mts_net_get_grads", "mts_net_forward", "mts_net_backward", "mts_net_adam", "mts_net_gemm_check",
@@ -112,6 +112,7 @@ def load_library(path=LIB_PATH):
     lib.mts_batch_destroy.argtypes = [C.c_void_p]
     lib.mts_batch_cells.argtypes = [C.c_void_p, _I64]
     lib.mts_batch_run.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_int, _I64, _D, _D, _I64]
+    lib.mts_batch_step.argtypes = [C.c_void_p, C.c_int]
     lib.mts_batch_activate.argtypes = [C.c_void_p, _I32]
     lib.mts_batch_fields.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
     lib.mts_batch_set_state.argtypes = [C.c_void_p, C.c_int, _D]
@@ -397,6 +398,11 @@ class Batch:
         _check(self.lib, self.lib.mts_batch_run(self.h, int(max_steps), int(check_every), float(tol), int(bool(graph)),
                                                 _p(steps, _I64), _p(sec, _D), _p(ux, _D), C.byref(upd)))
         return dict(steps=steps, seconds=sec, ux=ux, updates=upd.value)
+
+    def step(self, nsteps):
+        """nsteps steps of the active lattices (mts_batch_step): no statistics,
+        no convergence test; asynchronous on the batch's stream."""
+        _check(self.lib, self.lib.mts_batch_step(self.h, int(nsteps)))
 
     def activate(self, mask):
         m = np.ascontiguousarray(mask, np.int32)

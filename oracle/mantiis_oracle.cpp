@@ -92,6 +92,11 @@ static const double Minv[81] = {
     a9, c18, b36, -d6, -e12, d6, e12, 0, -q4,
     a9, c18, b36, -d6, -e12, -d6, -e12, 0, q4,
     a9, c18, b36, d6, e12, -d6, -e12, 0, -q4};
+// The same matrices as the reference's global std::vectors (_GLOBAL_::M,
+// _GLOBAL_::Minv), which the collisions read through matMulLin: a product
+// whose operands may alias the output keeps its accumulator in memory, the
+// cost the reference's loops carry.
+static const std::vector<double> MmV(Mm, Mm + 81), MinvV(Minv, Minv + 81);
 
 // ---------------------------------------------------------------- globals
 // globalParameters.hpp:45-222 (_GLOBAL_ namespace + recompute()).
@@ -469,26 +474,33 @@ struct Sim {
     }
 
     // ---- coreLB.hpp:420-430
+    // (std::pow as the reference writes it: pow(x, 2) is x * x exactly)
     void equilibrium() {
 #pragma omp parallel for
         for (int64_t k = 0; k < G.size; k++)
             for (int ic = 0; ic < NC; ic++) {
                 double cu = EX[ic] * ux[k] + EY[ic] * uy[k];
-                feq[k * NC + ic] = W[ic] * rho[k] * (1.0 + F1 * cu + F2 * (cu * cu) - F3 * usq[k]);
+                feq[k * NC + ic] = W[ic] * rho[k] * (1.0 + F1 * cu + F2 * std::pow(cu, 2) - F3 * usq[k]);
             }
     }
 
-    // ---- coreLB.hpp:634-643 (accumulates into mat3)
-    static void matMulLin(const double* m1, const double* m2, double* m3) {
+    // ---- coreLB.hpp:634-643 (accumulates into mat3).  The collisions keep
+    // the reference's containers: heap vectors for the rate matrix and the
+    // four work vectors of each thread, refilled per cell -- the same
+    // arithmetic as plain arrays, and the cost the timed CPU baseline should
+    // carry (the reference's collisions are 65-82% of its transport step).
+    template <class A, class B, class Out>
+    static void matMulLin(const A& m1, const B& m2, Out& m3) {
         for (int u = 0; u < NC; u++)
             for (int v = 0; v < NC; v++) m3[u] += m1[NC * u + v] * m2[v];
     }
     // MRT rates diag(1, 1/1.1, 1/1.2, 1/td, 1/tq, 1/td, 1/tq, 1/ts, 1/ts) as a
     // dense matrix (coreLB.hpp:705-715, 760-770, 599-609).
-    static void fillR(double* R, double tau_d, double tau_q, double tau_s) {
-        for (int u = 0; u < 81; u++) R[u] = 0;
-        double d[9] = {1.0 / 1.0, 1.0 / 1.1, 1.0 / 1.2, 1.0 / tau_d, 1.0 / tau_q,
-                       1.0 / tau_d, 1.0 / tau_q, 1.0 / tau_s, 1.0 / tau_s};
+    template <class Mat>
+    static void fillR(Mat& R, double tau_d, double tau_q, double tau_s) {
+        for (int u = 0; u < 81; u++) R[u] = 0.0;
+        const double d[9] = {1.0 / 1.0, 1.0 / 1.1, 1.0 / 1.2, 1.0 / tau_d, 1.0 / tau_q,
+                             1.0 / tau_d, 1.0 / tau_q, 1.0 / tau_s, 1.0 / tau_s};
         for (int u = 0; u < 9; u++) R[u * 10] = d[u];
     }
     double tauQ(int64_t k) const {
@@ -501,18 +513,18 @@ struct Sim {
 
     // ---- coreLB.hpp:646-675 (all cells, every call)
     void regularize() {
-#pragma omp parallel for
+        std::vector<double> P(4, 0.0), H(4, 0.0), ccr(4, 0.0);
+#pragma omp parallel for firstprivate(P, H, ccr)
         for (int64_t k = 0; k < G.size; k++) {
-            double P[4] = {0, 0, 0, 0};
+            std::fill(P.begin(), P.end(), 0.0);
             for (int ic = 0; ic < NC; ic++) {
-                double ccr[4] = {(double)(EX[ic] * EX[ic]), (double)(EX[ic] * EY[ic]),
-                                 (double)(EX[ic] * EY[ic]), (double)(EY[ic] * EY[ic])};
+                ccr = {(double)(EX[ic] * EX[ic]), (double)(EX[ic] * EY[ic]), (double)(EX[ic] * EY[ic]),
+                       (double)(EY[ic] * EY[ic])};
                 double d = f[k * NC + ic] - feq[k * NC + ic];
                 for (int u = 0; u < 4; u++) P[u] += d * ccr[u];
             }
             for (int ic = 0; ic < NC; ic++) {
-                double H[4] = {EX[ic] * EX[ic] / CS2 - 1, EX[ic] * EY[ic] / CS2, EX[ic] * EY[ic] / CS2,
-                               EY[ic] * EY[ic] / CS2 - 1};
+                H = {EX[ic] * EX[ic] / CS2 - 1, EX[ic] * EY[ic] / CS2, EX[ic] * EY[ic] / CS2, EY[ic] * EY[ic] / CS2 - 1};
                 fneq[k * NC + ic] = W[ic] * (1. / (2. * CS2)) * (P[0] * H[0] + P[1] * H[1] + P[2] * H[2] + P[3] * H[3]);
             }
         }
@@ -528,30 +540,36 @@ struct Sim {
     }
     // ---- coreLB.hpp:694-742
     void MRT(int lvl) {
-#pragma omp parallel for
+        std::vector<double> R(81), t1(NC), t2(NC), t3(NC), t4(NC);
+#pragma omp parallel for firstprivate(R, t1, t2, t3, t4)
         for (int64_t k = 0; k < G.size; k++) {
             if (G.gridLevel[k] != lvl) continue;
-            double R[81], t1[9], t2[9] = {0}, t3[9] = {0}, t4[9] = {0};
+            std::fill(t2.begin(), t2.end(), 0.0);
+            std::fill(t3.begin(), t3.end(), 0.0);
+            std::fill(t4.begin(), t4.end(), 0.0);
             fillR(R, tauD(k), tauQ(k), taus[k]);
             for (int ic = 0; ic < NC; ic++) t1[ic] = f[k * NC + ic] - feq[k * NC + ic];
-            matMulLin(Mm, t1, t2);
+            matMulLin(MmV, t1, t2);
             matMulLin(R, t2, t3);
-            matMulLin(Minv, t3, t4);
+            matMulLin(MinvV, t3, t4);
             for (int ic = 0; ic < NC; ic++) f[k * NC + ic] = f[k * NC + ic] - t4[ic];
         }
     }
     // ---- coreLB.hpp:744-792
     void RMRT(int lvl) {
         regularize();
-#pragma omp parallel for
+        std::vector<double> R(81), t1(NC), t2(NC), t3(NC), t4(NC);
+#pragma omp parallel for firstprivate(R, t1, t2, t3, t4)
         for (int64_t k = 0; k < G.size; k++) {
             if (G.gridLevel[k] != lvl) continue;
-            double R[81], t1[9], t2[9] = {0}, t3[9] = {0}, t4[9] = {0};
+            std::fill(t2.begin(), t2.end(), 0.0);
+            std::fill(t3.begin(), t3.end(), 0.0);
+            std::fill(t4.begin(), t4.end(), 0.0);
             fillR(R, 1.0, tauQ(k), taus[k]);
             for (int ic = 0; ic < NC; ic++) t1[ic] = fneq[k * NC + ic];
-            matMulLin(Mm, t1, t2);
+            matMulLin(MmV, t1, t2);
             matMulLin(R, t2, t3);
-            matMulLin(Minv, t3, t4);
+            matMulLin(MinvV, t3, t4);
             for (int ic = 0; ic < NC; ic++) f[k * NC + ic] = feq[k * NC + ic] + fneq[k * NC + ic] - t4[ic];
         }
     }
@@ -569,6 +587,7 @@ struct Sim {
     }
     // ---- coreLB.hpp:834-856 (push; the MPI exchange is empty at np = 1)
     void stream(int lvl) {
+#pragma omp parallel for
         for (size_t i = 0; i < streamID.size(); i++)
             if (G.gridLevel[streamID[i]] == lvl)
                 f[streamLOC[i] * NC + streamIC[i]] = ftemp[streamID[i] * NC + streamIC[i]];
@@ -1176,6 +1195,14 @@ int orc_threads() {
     return omp_get_max_threads();
 #else
     return 1;
+#endif
+}
+// OpenMP threads of the -Ofast build (the reference's OMP_NUM_THREADS)
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
 #endif
 }
 }  // extern "C"

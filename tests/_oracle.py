@@ -70,6 +70,7 @@ def load(fast=False):
     lib.orc_nearest_set.restype = C.c_int64
     lib.orc_nearest_set.argtypes = [_U8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
     lib.orc_downsample.argtypes = [_I, C.c_int, C.c_int, _I]
+    lib.orc_set_threads.argtypes = [C.c_int]
     return lib
 
 

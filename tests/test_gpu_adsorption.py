@@ -111,3 +111,38 @@ def test_non_square_lattice():
     assert np.isfinite(da).all() and np.isfinite(db).all()
     assert abs(da[0] - m0) / m0 < 1e-12 and abs(db[0] - m0) / m0 < 1e-6
     assert rel_l2(b.get("ux"), a.get("ux")) < 1e-3
+
+
+def test_shanchen_fp32_across_tile_columns():
+    """The FP32 Shan-Chen pass runs one block per 128 x 8 site tile and keeps a
+    cell's column as x mod 128 in its cell word.  Lattices wider than one tile
+    column, with sides that are multiples of neither 128 nor 8, exercise tile
+    columns tx >= 1, the ring of sites two tile columns share and partial
+    edge tiles: FP64 library vs oracle on a 300 x 300 torus (3 tile columns),
+    FP32 vs FP64 there and on a 300 x 90 torus, 50 steps each."""
+    import _oracle as O
+    from mantiis_amd import geometry as G
+    from mantiis_amd.hip import Lattice
+    for nx, ny in ((300, 300), (300, 90)):
+        banner(f"adsorption {nx}x{ny} torus fp32dev vs fp64 (and fp64 vs oracle when square), 50 steps")
+        pore = G.sanitize(G.disk_packing(300, 0.6, 3, 8, seed=8)[:ny, :nx].copy())
+        kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+        a = Lattice(pore, kn, lp, precision="fp64", collision=1, wall=0, periodic_y=1, **ADSORPTION)
+        b = Lattice(pore, kn, lp, precision="fp32dev", collision=1, wall=0, periodic_y=1, **ADSORPTION)
+        a.step("adsorption", 50)
+        b.step("adsorption", 50)
+        if nx == ny:
+            o = O.Oracle(pore, kn, lp, periodic_y=True, **ADSORPTION)
+            o.set_operators(1, 0)
+            o.run("adsorption", 50, 0.0)
+            for name in ("f", "rho", "ux", "uy", "psi"):
+                d = np.abs(a.get(name) - o.get(name)).max()
+                print(name, "fp64 vs oracle max-abs", d)
+                assert d <= 1e-15 * max(1.0, np.abs(o.get(name)).max()), name
+            o.close()
+        for name, tol in (("rho", 1e-6), ("ux", 1e-4), ("uy", 1e-4), ("psi", 1e-6)):
+            e = rel_l2(b.get(name), a.get(name))
+            print(f"{nx}x{ny}", name, "fp32 vs fp64 rel-l2", e)
+            assert e < tol, (name, e)
+        a.close()
+        b.close()

@@ -6,6 +6,7 @@ import pytest
 import _oracle as O
 from _gpu_util import TRANSPORT, banner, pair, rel_l2
 from mantiis_amd import geometry as G
+from mantiis_amd.hip import Lattice
 
 pytestmark = pytest.mark.gpu
 
@@ -154,3 +155,27 @@ def test_mass_conservation_long_run(prec):
     assert np.isfinite(d).all()
     assert abs(d[0] - m0) / m0 < (1e-12 if prec == "fp64" else 1e-6)
     assert d[1] > 0  # the body force drives a net +x flux
+
+
+@pytest.mark.parametrize("kind,prec", [("transport", "fp64"), ("transport", "fp32dev"), ("adsorption", "fp64"),
+                                       ("adsorption", "fp32dev")])
+def test_nonfinite_state_is_reported(kind, prec):
+    """A NaN planted in f (through mts_set_field) makes the fused step call
+    return MTS_ERR_NONFINITE (-8) instead of running on: the reference's test
+    diff < tol is never true for NaN, so a diverged run would use all its
+    iterations."""
+    banner(f"NaN injection {kind} {prec}")
+    from mantiis_amd import geometry as G
+    from mantiis_amd.hip import MtsError
+    pore = G.disk_packing(64, 0.7, 3, 8, seed=2)
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    coll, wall = (2, 2) if kind == "transport" else (1, 0)
+    g = Lattice(pore, kn, lp, precision=prec, collision=coll, wall=wall, periodic_y=1, dx=1e-9, Fbody=1e-6)
+    g.step(kind, 2)
+    f = g.get("f")
+    f[9 * 100 + 3] = np.nan
+    g.set("f", f)
+    with pytest.raises(MtsError) as e:
+        g.step(kind, 4)
+    assert e.value.code == -8, e.value
+    g.close()
