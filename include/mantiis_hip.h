@@ -372,6 +372,21 @@ int mts_shape_run(mts_shape h, const uint8_t* domain, double resolution, double 
                   uint8_t* skeleton, int* iterations);
 int mts_shape_stream(mts_shape h, void** stream);
 
+/* Allocation-free form of mts_shape_run (no reference counterpart: for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture elsewhere in their pipeline, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same inputs and outputs as mts_shape_run, given
+ * directly (device domain, device outputs, sides nx, ny in [2, 512]); the
+ * work buffers live in one static device array of the library (8.3 MB per
+ * GPU, held until the process ends) instead of a handle, and the run goes on
+ * the caller's stream (NULL: the null stream), which it synchronizes.  It
+ * calls no hipMalloc / hipFree.  Sides beyond 512 return MTS_ERR_ARG.  Every
+ * call on one GPU uses that GPU's one array: calls that overlap on one GPU
+ * (from several host threads or streams) race and must be serialized by the
+ * caller. */
+int mts_shape_run_static(int nx, int ny, const uint8_t* domain, double resolution, double mfp, double* locpore,
+                         double* kn, uint8_t* skeleton, int* iterations, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

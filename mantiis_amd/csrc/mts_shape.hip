@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -457,8 +458,82 @@ __global__ void k_props(const uint8_t* __restrict__ dom, const uint8_t* __restri
     kn[p] = mfp / lp / resolution;
 }
 
+// Fills n bytes with v (the flags and counters of a run are cleared by a
+// kernel rather than a memset node).
+__global__ void k_fill8(uint8_t* __restrict__ p, int64_t n, uint8_t v) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
 int grid_of(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
+// Work buffers of one run for sides nx x ny, as byte offsets into one
+// region (each 256-byte aligned): the handle's buffers in the order
+// mts_shape_create allocates them.
+struct ShapeLayout {
+    int m, H, Wc, WW, nseg, tilesX, tilesY;
+    size_t cv0, cv1, skel, first, last, g, k2, zrow, nearest, cnt, chg, err, total;
+};
+constexpr size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+constexpr ShapeLayout shape_layout(int nx, int ny) {
+    ShapeLayout L{};
+    L.m = nx > ny ? nx : ny;
+    L.H = ny + 2 * L.m;
+    L.Wc = nx + 2 * L.m;
+    L.WW = (L.Wc + 63) / 64;
+    L.nseg = (ny + SEG - 1) / SEG;
+    L.tilesX = (L.WW + THIN_TW - 1) / THIN_TW;
+    L.tilesY = (L.H + THIN_TH - 1) / THIN_TH;
+    const size_t np = (size_t)nx * ny, nw = (size_t)L.H * L.WW;
+    size_t o = 0;
+    L.cv0 = o, o = align256(o + nw * sizeof(u64));
+    L.cv1 = o, o = align256(o + nw * sizeof(u64));
+    L.skel = o, o = align256(o + np);
+    L.first = o, o = align256(o + 2 * (size_t)L.nseg * nx * sizeof(int));
+    L.last = o, o = align256(o + 2 * (size_t)L.nseg * nx * sizeof(int));
+    L.g = o, o = align256(o + 2 * np * sizeof(int));
+    L.k2 = o, o = align256(o + 2 * np * sizeof(int));
+    L.zrow = o, o = align256(o + 2 * np * sizeof(int));
+    L.nearest = o, o = align256(o + np * sizeof(int));
+    L.cnt = o, o = align256(o + CHECK_EVERY * sizeof(unsigned));
+    L.chg = o, o = align256(o + 2 * (size_t)L.tilesX * L.tilesY);
+    L.err = o, o = align256(o + sizeof(int));
+    L.total = o;
+    return L;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Work memory of mts_shape_run_static: one file-scope device array per GPU,
+// sized for the largest sides that entry point accepts.  Host code never
+// uses the name `scratch` as an address: the device address of each GPU's
+// copy comes from hipGetSymbolAddress, once per device, and is kept below.
+constexpr int SHAPE_STATIC_MAX_SIDE = 512;
+constexpr size_t SHAPE_SCRATCH_BYTES = shape_layout(SHAPE_STATIC_MAX_SIDE, SHAPE_STATIC_MAX_SIDE).total;
+static __device__ __align__(256) unsigned char scratch[SHAPE_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of the current device's scratch array.
+int scratch_of_current_device(unsigned char** out) {
+    int dev = 0;
+    SH_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return set_error(MTS_ERR_ARG, "mts_shape_run_static: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(MTS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MTS_OK;
+}
 }  // namespace
 
 struct mts_shape_s {
@@ -476,6 +551,7 @@ struct mts_shape_s {
     uint8_t* chg = nullptr;  // two arrays of per-tile change flags
     int tilesX = 0, tilesY = 0;
     int* err = nullptr;
+    bool own_stream = true;  // false: the caller's stream (mts_shape_run_static)
     std::vector<void*> allocs;
     int64_t launches = 0;  // thinning launches of the last run
 
@@ -491,7 +567,7 @@ struct mts_shape_s {
     ~mts_shape_s() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* q : allocs) (void)hipFree(q);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -554,7 +630,7 @@ int mts_shape_run(mts_shape h, const uint8_t* domain, double resolution, double 
     const int nx = h->nx, ny = h->ny;
     const int64_t np = (int64_t)nx * ny, nw = (int64_t)h->H * h->WW;
     hipStream_t s = h->stream;
-    SH_TRY(hipMemsetAsync(h->err, 0, sizeof(int), s));
+    k_fill8<<<1, 64, 0, s>>>((uint8_t*)h->err, sizeof(int), 0);
     k_canvas<<<grid_of(nw, 256), 256, 0, s>>>(domain, nx, ny, h->m, h->H, h->Wc, h->WW, h->cv[0], h->err);
     SH_TRY(hipGetLastError());
     int herr = 0;
@@ -566,13 +642,13 @@ int mts_shape_run(mts_shape h, const uint8_t* domain, double resolution, double 
     const int ntiles = h->tilesX * h->tilesY;
     const dim3 tg(h->tilesX, h->tilesY);
     const int64_t maxLaunches = nw * 64 / (THIN_S / 2) + 2;  // each iteration but the last deletes a pixel
-    SH_TRY(hipMemsetAsync(h->chg, 1, ntiles, s));            // every tile is live in the first launch
+    k_fill8<<<grid_of(ntiles, 256), 256, 0, s>>>(h->chg, ntiles, 1);  // every tile is live in the first launch
     int cur = 0;
     bool done = false;
     h->launches = 0;
     while (!done) {
         if (h->launches >= maxLaunches) return set_error(MTS_ERR_STATE, "mts_shape_run: thinning did not settle");
-        SH_TRY(hipMemsetAsync(h->cnt, 0, CHECK_EVERY * sizeof(unsigned), s));
+        k_fill8<<<1, 64, 0, s>>>((uint8_t*)h->cnt, CHECK_EVERY * sizeof(unsigned), 0);
         for (int k = 0; k < CHECK_EVERY; k++) {
             const int64_t l = h->launches + k;
             k_thin<<<tg, THIN_THREADS, 0, s>>>(h->cv[cur], h->cv[cur ^ 1], h->H, h->Wc, h->WW, h->tilesX, h->tilesY,
@@ -638,6 +714,45 @@ int mts_shape_properties(const int32_t* domain, int nx, int ny, double resolutio
     if (rc == MTS_OK && e != hipSuccess) rc = set_error(MTS_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
     mts_shape_destroy(h);
     return rc;
+}
+
+int mts_shape_run_static(int nx, int ny, const uint8_t* domain, double resolution, double mfp, double* locpore,
+                         double* kn, uint8_t* skeleton, int* iterations, void* stream) {
+    if (!domain || !locpore || !kn) return set_error(MTS_ERR_ARG, "mts_shape_run_static: null argument");
+    if (nx < 2 || ny < 2 || nx > SHAPE_STATIC_MAX_SIDE || ny > SHAPE_STATIC_MAX_SIDE)
+        return set_error(MTS_ERR_ARG, "mts_shape_run_static: sides must lie in [2, " +
+                                          std::to_string(SHAPE_STATIC_MAX_SIDE) + "] (the static work memory holds " +
+                                          std::to_string(SHAPE_SCRATCH_BYTES) + " bytes)");
+    const ShapeLayout L = shape_layout(nx, ny);
+    if (L.total > SHAPE_SCRATCH_BYTES) return set_error(MTS_ERR_ARG, "mts_shape_run_static: sides too large");
+    unsigned char* base = nullptr;
+    if (int rc = scratch_of_current_device(&base)) return rc;
+    // a handle view over the scratch array: no allocation, the caller's stream
+    mts_shape_s h;
+    h.own_stream = false;
+    h.stream = (hipStream_t)stream;
+    h.nx = nx;
+    h.ny = ny;
+    h.m = L.m;
+    h.H = L.H;
+    h.Wc = L.Wc;
+    h.WW = L.WW;
+    h.nseg = L.nseg;
+    h.tilesX = L.tilesX;
+    h.tilesY = L.tilesY;
+    h.cv[0] = (u64*)(base + L.cv0);
+    h.cv[1] = (u64*)(base + L.cv1);
+    h.skel = base + L.skel;
+    h.first = (int*)(base + L.first);
+    h.last = (int*)(base + L.last);
+    h.g = (int*)(base + L.g);
+    h.k2 = (int*)(base + L.k2);
+    h.zrow = (int*)(base + L.zrow);
+    h.nearest = (int*)(base + L.nearest);
+    h.cnt = (unsigned*)(base + L.cnt);
+    h.chg = base + L.chg;
+    h.err = (int*)(base + L.err);
+    return mts_shape_run(&h, domain, resolution, mfp, locpore, kn, skeleton, iterations);
 }
 
 }  // extern "C"

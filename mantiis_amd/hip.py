@@ -37,7 +37,7 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_batch_stream", "mts_batch_tune", "mts_batch_set_config", "mts_dev_alloc", "mts_dev_free", "mts_dev_upload", "mts_dev_download",
            "mts_net_create", "mts_net_destroy", "mts_net_param_count", "mts_net_get_params", "mts_net_set_params",
            "mts_net_get_grads", "mts_net_forward", "mts_net_backward", "mts_net_adam", "mts_net_gemm_check",
-           "mts_net_stream", "mts_shape_create", "mts_shape_destroy", "mts_shape_stream", "mts_shape_run",
+           "mts_net_stream", "mts_shape_create", "mts_shape_destroy", "mts_shape_stream", "mts_shape_run", "mts_shape_run_static",
            "mts_shape_properties"]
 
 
@@ -577,6 +577,24 @@ def shape_properties(domain, resolution, mfp, skeleton=False):
     _check(lib, lib.mts_shape_properties(_p(d, _I32), nx, ny, float(resolution), float(mfp), _p(lp, _D), _p(kn, _D),
                                          sk.ctypes.data_as(u8) if skeleton else None))
     return (lp, kn, sk) if skeleton else (lp, kn)
+
+
+def shape_run_static(domain, resolution, mfp, locpore, kn, skeleton=None, stream=None):
+    """mts_shape_run_static: the device-resident calculateProperties with no
+    handle and no device allocation (the library's static work memory, sides
+    up to 512).  domain, locpore, kn (and skeleton) are DeviceArrays of one
+    [ny][nx] shape; stream a hipStream_t (None: the null stream).  Returns
+    the thinning iterations run."""
+    lib = load_library()
+    lib.mts_shape_run_static.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.c_void_p]
+    ny, nx = domain.shape
+    it = C.c_int(0)
+    _check(lib, lib.mts_shape_run_static(int(nx), int(ny), domain.data_ptr(), float(resolution), float(mfp),
+                                         locpore.data_ptr(), kn.data_ptr(),
+                                         skeleton.data_ptr() if skeleton is not None else None, C.byref(it),
+                                         stream))
+    return it.value
 
 
 class ShapeProperties:
