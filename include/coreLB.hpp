@@ -175,6 +175,20 @@ public:
         check(mts_complete_domain(handle(), a.data(), b.data(), c.data()));
         return std::make_tuple(a, b, c);
     }
+    // FP64 permeability report (no reference counterpart): Darcy velocity
+    // U_D = sum u_x / (NX NY) over completeSingleGridDomain's pixels, mean u_x
+    // of the fluid pixels, sum rho u_x, sum rho, porosity, the finest level's
+    // mean tau_s, k = nu U_D / Fbody (lattice units, nu = (tau - 0.5) / 3) and
+    // k Cl^2 [m^2].
+    struct Permeability {
+        double darcy_velocity, mean_ux, flux, mass, porosity, tau, k_lattice, k_m2;
+    };
+    Permeability permeability() {
+        double o[8];
+        check(mts_permeability(handle(), o));
+        return Permeability{o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7]};
+    }
+
     // coreLB.hpp:1619-1641: f re-expanded to NX*NY*9 with zeros at solids.
     std::vector<double> prepareDistributions() {
         const std::vector<double>& fc = f.host();

@@ -243,11 +243,30 @@ public:
                 IO::writeBinary(folder + "uy.bin", std::get<1>(d));
                 IO::writeBinary(folder + "rho.bin", std::get<2>(d));
             }
+            // FP64 permeability report (no reference counterpart), for the
+            // flow runs: permeability.txt, one "key value" per line
+            LB2D::Permeability perm{};
+            const bool flow = simulation_type != "adsorption";
+            if (flow) {
+                perm = lb->permeability();
+                std::ofstream pf(folder + "permeability.txt");
+                pf << std::setprecision(15);
+                pf << "darcy_velocity " << perm.darcy_velocity << "\n";
+                pf << "mean_ux " << perm.mean_ux << "\n";
+                pf << "flux " << perm.flux << "\n";
+                pf << "mass " << perm.mass << "\n";
+                pf << "porosity " << perm.porosity << "\n";
+                pf << "tau " << perm.tau << "\n";
+                pf << "k_lattice " << perm.k_lattice << "\n";
+                pf << "k_m2 " << perm.k_m2 << "\n";
+                std::cout << "Permeability (m^2): " << std::setprecision(15) << perm.k_m2 << "\n";
+            }
             std::ofstream details(folder + "SimulationDetails.txt");
             details << "Domain dimension: " << shape->Nx << " x " << shape->Ny << "\n";
             details << "Tolerance was set to " << tol << "\n";
             details << "Number timesteps executed: " << lb->t << "\n";
             details << "The number of active cells: " << grid->globalGridSize << "\n";
+            if (flow) details << "Permeability (m^2): " << std::setprecision(15) << perm.k_m2 << "\n";
             const auto ms = duration_cast<milliseconds>(high_resolution_clock::now() - start).count();
             std::cout << "Total run time (milliseconds) " << std::setprecision(15) << ms << std::endl;
             details << "Total run time (milliseconds): " << std::setprecision(15) << ms;

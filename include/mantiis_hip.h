@@ -74,6 +74,11 @@ typedef struct mts_params {
                       *    rule), y-seam-only links dropped (default);
                       * 1: torus -- links across either seam are classified
                       *    like interior links (mass-conserving)            */
+    int batch_steps; /* batched solver (mts_batch_*): time steps per launch --
+                      * 2: the tile kernels, two steps per launch staged in
+                      *    LDS (default; every side must be a multiple of 32,
+                      *    else the batch runs one step per launch);
+                      * 1: one step per launch in cell order               */
 } mts_params;
 
 /* Fills the defaults above. */
@@ -223,6 +228,15 @@ int mts_set_field(mts_sim s, int field, const double* in);
 /* LB2D::completeSingleGridDomain (coreLB.hpp:1344-1411): ux, uy, rho as
  * nx*ny row-major arrays with solid pixels set to 0. */
 int mts_complete_domain(mts_sim s, double* ux, double* uy, double* rho);
+
+/* FP64 permeability report of the current state (no reference counterpart;
+ * the flux integration a user of MantiisApp computes from ux.txt): over the
+ * nx*ny pixels of completeSingleGridDomain (coreLB.hpp:1344-1411), out8 =
+ * {U_D = sum u_x / (nx ny) (Darcy, superficial), mean u_x of the fluid
+ * pixels, sum rho u_x, sum rho, porosity, tau = mean tau_s of the finest
+ * level's cells, k = nu U_D / Fbody with nu = (tau - 0.5) / 3 (lattice
+ * units), k dx^2 [m^2]}; sums in FP64.  k is 0 when Fbody is 0. */
+int mts_permeability(mts_sim s, double* out8);
 
 /* FP64 diagnostics of the current state (no reference counterpart):
  * out4 = {sum rho, sum j_x, sum j_y, fluid cell count}. */

@@ -63,6 +63,9 @@ struct mts_sim_s {
     int32_t* d_cellPos = nullptr;
     void* d_A = nullptr;
     void* d_B = nullptr;
+    // MDBB with FP32 populations: the collided state of the wall cells, 4
+    // planes per buffer, read by a pass's gather and written by its collision
+    float* d_M[2] = {nullptr, nullptr};
     // FP32 carried populations live in cell order (single block)
     CompactImage cimg_h;
     std::vector<float> cpar_h;
@@ -269,6 +272,7 @@ static int upload_cpar(mts_sim_s* s, const std::vector<CellParam>& params, const
         o[2] = (float)q.s_d;
         o[3] = (float)q.r_srbb;
         o[4] = (float)(q.r_srbb - (double)o[3]);
+        o[5] = (float)q.r_mdbb;
     }
     float* cpar;
     int rc;
@@ -978,6 +982,7 @@ void mts_default_params(mts_params* p) {
     p->multiblock = 0;
     p->device = 0;
     p->periodic_y = 0;
+    p->batch_steps = 2;
 }
 
 const char* mts_last_error(void) { return g_err.c_str(); }
@@ -1367,33 +1372,43 @@ int mts_step_transport(mts_sim s, int nsteps) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
     if (nsteps <= 0) return fail(MTS_ERR_ARG, "nsteps must be positive");
     if (s->multi) return multi_transport(s, nsteps);
-    if (s->prm.wall == MTS_WALL_MDBB) {
-        if (s->prm.precision != MTS_FP64) return fail(MTS_ERR_UNSUPPORTED, "MDBB wall is provided in FP64 only");
+    const bool mdbb = s->prm.wall == MTS_WALL_MDBB;
+    if (mdbb && s->prm.precision == MTS_FP64) {  // FP64: the per-method kernels
         for (int i = 0; i < nsteps; i++) {
             int rc = transport_step_phases(s);
             if (rc) return rc;
         }
         return MTS_OK;
     }
+    int rc;
+    if (mdbb && !s->d_M[0]) {
+        const size_t nm = (size_t)4 * s->cimg.cstride;
+        if ((rc = s->alloc(&s->d_M[0], nm)) || (rc = s->alloc(&s->d_M[1], nm))) return rc;
+    }
     LaunchCtx c = s->ctx();
     TransportSpec t;
-    int rc;
+    int mc = 0;  // MDBB state: written by a pass's collision, read by the next pass's gather
     // pass 0: collide the canonical state
     t.fcan = s->d_f;
     t.Aout = s->d_A;
     t.usq_old = nsteps == 1 ? s->d_usq_old : nullptr;
+    t.Mout = s->d_M[0];
     if ((rc = launch_transport(c, 0, t))) return fail(rc, "operator combination has no fused kernel");
     for (int i = 1; i < nsteps; i++) {
         TransportSpec u;
         u.Ain = s->d_A;
         u.Aout = s->d_B;
         u.usq_old = (i == nsteps - 1) ? s->d_usq_old : nullptr;
+        u.Min = s->d_M[mc];
+        u.Mout = s->d_M[mc ^ 1];
         s->tick(c);
         launch_transport(c, 1, u);
         s->tock(c);
         std::swap(s->d_A, s->d_B);
+        mc ^= 1;
     }
     TransportSpec z;
+    z.Min = s->d_M[mc];
     z.Ain = s->d_A;
     z.fcan = s->d_f;
     z.rho = s->d_rho;
@@ -1642,6 +1657,45 @@ int mts_complete_domain(mts_sim s, double* ux, double* uy, double* rho) {
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return MTS_OK;
+}
+
+int mts_permeability(mts_sim s, double* out8) {
+    if (!s || !out8) return fail(MTS_ERR_ARG, "null argument");
+    const int nx = s->multi ? s->mg.nx : s->geo.nx, ny = s->multi ? s->mg.ny : s->geo.ny;
+    const size_t P = (size_t)nx * ny;
+    std::vector<double> ux(P), rho(P);
+    int rc;
+    if ((rc = mts_complete_domain(s, ux.data(), nullptr, rho.data()))) return rc;
+    std::vector<uint8_t> fluid(P, 0);
+    if (s->multi)
+        for (size_t p = 0; p < P; p++) fluid[p] = s->mg.pore[p] == 1;
+    else
+        for (int32_t p : s->geo.cellPos) fluid[p] = 1;
+    double su = 0, sj = 0, sr = 0;
+    int64_t nf = 0;
+    for (size_t p = 0; p < P; p++) {
+        if (!fluid[p]) continue;
+        su += ux[p];
+        sj += rho[p] * ux[p];
+        sr += rho[p];
+        nf++;
+    }
+    // mean relaxation time of the finest level's cells (the report's units)
+    const std::vector<double>& taus = s->multi ? s->mg.taus : s->geo.taus;
+    double st = 0;
+    int64_t nt = 0;
+    for (size_t c = 0; c < taus.size(); c++) {
+        if (s->multi && (s->mg.level[c] != 1 || s->mg.isBuffer[c])) continue;
+        st += taus[c];
+        nt++;
+    }
+    const double UD = su / (double)P, tau = nt ? st / (double)nt : 0.5, nu = (tau - 0.5) / 3.0;
+    const double F = s->Fbody;
+    const double k = F != 0.0 ? nu * UD / F : 0.0;
+    const double out[8] = {UD, nf ? su / (double)nf : 0.0, sj, sr, (double)nf / (double)P, tau, k,
+                           k * s->ph.Cl * s->ph.Cl};
+    std::copy(out, out + 8, out8);
+    return std::isfinite(UD) ? MTS_OK : fail(MTS_ERR_NONFINITE, "velocity field is not finite");
 }
 
 int mts_diagnostics(mts_sim s, double* out4) {

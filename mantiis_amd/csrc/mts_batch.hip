@@ -46,6 +46,9 @@ struct mts_batch_s {
     std::vector<BatBlock> hblocks;
     std::vector<std::vector<int32_t>> blocks_of;  // per lattice: its block ids
     float* A[2] = {nullptr, nullptr};
+    // MDBB: the collided state of the wall cells beside A[cur] (4 planes)
+    float* M[2] = {nullptr, nullptr};
+    float* msnap = nullptr;
     int cur = 0;
     float* mac = nullptr;
     double* acc = nullptr;
@@ -55,6 +58,20 @@ struct mts_batch_s {
     int32_t* d_queue = nullptr;
     int32_t* d_nactive = nullptr;
     int nactive = 0;
+    // tile kernels (two steps per launch): tile-major cell order, per tile
+    // row the fluid mask and first cell, per tile its 3 x 3 neighbourhood
+    bool tiled = false;
+    int ntiles = 0;
+    std::vector<std::vector<int32_t>> tiles_of;  // per lattice: tiles holding a cell
+    std::vector<int32_t> tfirst;                 // per lattice: first tile (nb + 1)
+    int32_t* d_tnb = nullptr;
+    uint32_t* d_tmask = nullptr;
+    int32_t* d_tpre = nullptr;
+    int32_t* d_tqueue = nullptr;
+    int32_t* d_tfirst = nullptr;
+    int32_t* d_ntactive = nullptr;
+    double* tpart = nullptr;  // per tile sums (zero for the tiles without a cell)
+    int ntactive = 0;
     std::vector<char> active;
     std::vector<double> u_prev;
     std::vector<void*> allocs;
@@ -107,12 +124,20 @@ struct mts_batch_s {
         return c;
     }
     int upload_queue() {
-        std::vector<int32_t> q;
+        std::vector<int32_t> q, tq;
         for (int g = 0; g < nb; g++)
-            if (active[g]) q.insert(q.end(), blocks_of[g].begin(), blocks_of[g].end());
+            if (active[g]) {
+                q.insert(q.end(), blocks_of[g].begin(), blocks_of[g].end());
+                if (tiled) tq.insert(tq.end(), tiles_of[g].begin(), tiles_of[g].end());
+            }
         nactive = (int)q.size();
         if (!q.empty()) BT_TRY(hipMemcpyAsync(d_queue, q.data(), q.size() * 4, hipMemcpyHostToDevice, stream));
         BT_TRY(hipMemcpyAsync(d_nactive, &nactive, 4, hipMemcpyHostToDevice, stream));
+        if (tiled) {
+            ntactive = (int)tq.size();
+            if (!tq.empty()) BT_TRY(hipMemcpyAsync(d_tqueue, tq.data(), tq.size() * 4, hipMemcpyHostToDevice, stream));
+            BT_TRY(hipMemcpyAsync(d_ntactive, &ntactive, 4, hipMemcpyHostToDevice, stream));
+        }
         return hipStreamSynchronize(stream) == hipSuccess ? MTS_OK : set_error(MTS_ERR_HIP, "queue upload");
     }
     // Grid of a step: the active blocks rounded up to a power of two (at
@@ -125,6 +150,58 @@ struct mts_batch_s {
         int g = 1024;
         while (g < n) g *= 2;
         return std::min<int>(g, (int)hblocks.size());
+    }
+    int tgrid_for(int n) const {
+        if (n <= 0) return 0;
+        int g = 1024;
+        while (g < n) g *= 2;
+        return std::min<int>(g, ntiles);
+    }
+    int grid_key() const { return tiled ? tgrid_for(ntactive) : grid_for(nactive); }
+    // tile kernels: one launch of ns (1 or 2) steps over the tile queue,
+    // A[cur] -> A[cur^1]; with stats the last post-stream state's fields and
+    // per-lattice sums
+    int advance(int ns, bool stats) {
+        TileLaunch T;
+        T.stream = stream;
+        T.coll = prm.collision;
+        T.wall = prm.wall;
+        T.nsteps = ns;
+        T.grid = tgrid_for(ntactive);
+        T.Ain = A[cur];
+        T.Aout = A[cur ^ 1];
+        T.tw = dev.cw;
+        T.pidx = dev.pidx;
+        T.cpar = dev.cpar;
+        T.tmask = d_tmask;
+        T.tpre = d_tpre;
+        T.tnb = d_tnb;
+        T.queue = d_tqueue;
+        T.nactive = d_ntactive;
+        T.mac = stats ? mac : nullptr;
+        T.part = stats ? tpart : nullptr;
+        T.S = dev.cstride;
+        T.rho0 = ph.rhoin;
+        T.Fbody = (float)prm.Fbody;
+        T.xrun = xrun;
+        T.one = params.size() == 1;
+        const int rc = launch_ctile(T);
+        if (stats) launch_bsum(tpart, d_tfirst, acc, nb, stream);
+        cur ^= 1;
+        return rc;
+    }
+    // nsteps plain steps (no statistics)
+    int steps(int nsteps) {
+        int rc;
+        if (tiled) {
+            for (int i = 0; i < nsteps / 2; i++)
+                if ((rc = advance(2, false))) return rc;
+            if (nsteps & 1) return advance(1, false);
+            return MTS_OK;
+        }
+        for (int i = 0; i < nsteps; i++)
+            if ((rc = step(false, 0))) return rc;
+        return MTS_OK;
     }
     // one step over the queue: A[cur] -> A[cur^1]
     int step(bool stats, int from) {
@@ -141,9 +218,12 @@ struct mts_batch_s {
         L.from_rest = from;
         L.threads = threads;
         L.xrun = xrun;
+        L.Min = M[cur];
+        L.Mout = M[cur ^ 1];
         if (from == 2) {  // the given state sits in A[cur^1]: read it, write A[cur]
             L.Ain = A[cur ^ 1];
             L.Aout = A[cur];
+            L.Mout = M[cur];
         }
         const int rc = launch_cbatch(L);
         if (stats) launch_bsum(part, d_first, acc, nb, stream);
@@ -153,6 +233,11 @@ struct mts_batch_s {
     // n steps (n even), the last with statistics
     int chunk_eager(int n) {
         int rc;
+        if (tiled) {
+            for (int i = 0; i < n / 2; i++)
+                if ((rc = advance(2, i == n / 2 - 1))) return rc;
+            return MTS_OK;
+        }
         for (int i = 0; i < n; i++)
             if ((rc = step(i == n - 1, 0))) return rc;
         return MTS_OK;
@@ -179,12 +264,19 @@ struct mts_batch_s {
             if (int rc = alloc(&snap, elems)) return rc;
         }
         const int cur0 = cur;
+        const size_t melems = (size_t)4 * dev.cstride;
+        if (M[0] && !msnap) {
+            if (int rc = alloc(&msnap, melems)) return rc;
+        }
         BT_TRY(hipMemcpyAsync(snap, A[cur0], elems * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        if (M[0]) BT_TRY(hipMemcpyAsync(msnap, M[cur0], melems * sizeof(float), hipMemcpyDeviceToDevice, stream));
         // puts the saved state back (also after a failed step: tuning never
         // leaves the batch partly advanced)
         auto restore = [&]() -> int {
             cur = cur0;
             if (hipMemcpyAsync(A[cur0], snap, elems * sizeof(float), hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+                (M[0] && hipMemcpyAsync(M[cur0], msnap, melems * sizeof(float), hipMemcpyDeviceToDevice, stream) !=
+                             hipSuccess) ||
                 hipStreamSynchronize(stream) != hipSuccess)
                 return set_error(MTS_ERR_HIP, "tune: restoring the state");
             return MTS_OK;
@@ -197,10 +289,11 @@ struct mts_batch_s {
         for (int k = 0; k < NCAND && rc == MTS_OK; k++) {
             threads = cand_threads(k);
             xrun = cand_xrun(k);
-            for (int i = 0; i < 2 && rc == MTS_OK; i++) rc = step(false, 0);  // first use of the variant
+            if (tiled && threads != 256) continue;  // the tile kernels run 256 threads per tile
+            rc = steps(2);                          // first use of the variant
             if (rc) break;
             if (hipEventRecord(e0, stream) != hipSuccess) rc = set_error(MTS_ERR_HIP, "tune: event");
-            for (int i = 0; i < each && rc == MTS_OK; i++) rc = step(false, 0);
+            rc = steps(each);
             if (rc) break;
             float ms = 0.0f;
             if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
@@ -229,7 +322,7 @@ struct mts_batch_s {
         return MTS_OK;
     }
     int chunk_graph(int n) {
-        if (!gexec || gchunk != n || gcur != cur || ggrid != grid_for(nactive)) {
+        if (!gexec || gchunk != n || gcur != cur || ggrid != grid_key()) {
             if (gexec) (void)hipGraphExecDestroy(gexec);
             if (graph) (void)hipGraphDestroy(graph);
             gexec = nullptr;
@@ -253,10 +346,11 @@ struct mts_batch_s {
             }
             gchunk = n;
             gcur = c0;
-            ggrid = grid_for(nactive);
+            ggrid = grid_key();
         }
         BT_TRY(hipGraphLaunch(gexec, stream));
-        if (n & 1) cur ^= 1;
+        const int flips = tiled ? n / 2 : n;  // launches in the chunk
+        if (flips & 1) cur ^= 1;
         return MTS_OK;
     }
 };
@@ -278,7 +372,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     else mts_default_params(&prm);
     if (prm.precision != MTS_FP32DEV) return set_error(MTS_ERR_UNSUPPORTED, "the batched solver runs FP32 deviations");
     if (prm.multiblock) return set_error(MTS_ERR_UNSUPPORTED, "the batched solver runs single-block lattices");
-    if (prm.collision < 0 || prm.collision > 2 || prm.wall < 0 || prm.wall > 1 + 1 || prm.wall == MTS_WALL_MDBB)
+    if (prm.collision < 0 || prm.collision > 2 || prm.wall < 0 || prm.wall > 2)
         return set_error(MTS_ERR_ARG, "operator combination has no batched kernel");
     mts_batch_s* b = new (std::nothrow) mts_batch_s();
     if (!b) return set_error(MTS_ERR_OOM, "host allocation");
@@ -301,6 +395,14 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     b->ncell.resize(nb);
     b->cellPos.resize(nb);
     b->blocks_of.resize(nb);
+    b->tiles_of.resize(nb);
+    b->tfirst.assign(nb + 1, 0);
+    // tile kernels when asked for and every side is a multiple of the tile
+    b->tiled = prm.batch_steps == 2 && prm.wall != MTS_WALL_MDBB;  // the tile kernels have no MDBB
+    for (int g = 0; g < nb && b->tiled; g++)
+        if (nx[g] % TILE || ny[g] % TILE) b->tiled = false;
+    std::vector<int32_t> htnb, htpre;
+    std::vector<uint32_t> htmask;
     CompactImage& C = b->cat;
     int64_t base = 0;
     for (int g = 0; g < nb; g++) {
@@ -326,7 +428,48 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
         C.rw.resize(base + n);
         C.pidx.resize(base + n);
         C.outm.resize(base + n);
-        for (int64_t c = 0; c < n; c++) {
+        if (b->tiled) {
+            // tile-major order: tiles in row-major tile order, in a tile its
+            // fluid sites in row-major order; per cell the link bits, the
+            // boundary type and whether the one-step pass applies the wall
+            // rule in link order
+            constexpr int TS = TILE;
+            const int ntx = nx[g] / TS, nty = ny[g] / TS, t0 = b->ntiles;
+            b->tfirst[g] = t0;
+            std::vector<int32_t> pos(n);
+            int64_t k = 0;
+            constexpr uint32_t keep = 0xffu | (0xffu << CW_WALL_SHIFT) | (63u << CW_TYPE_SHIFT) | CW_COMPLEX;
+            for (int ty = 0; ty < nty; ty++)
+                for (int tx = 0; tx < ntx; tx++) {
+                    const int t = t0 + ty * ntx + tx;
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++)
+                            htnb.push_back(t0 + ((ty + dy + nty) % nty) * ntx + (tx + dx + ntx) % ntx);
+                    const int64_t k0 = k;
+                    for (int ly = 0; ly < TS; ly++) {
+                        uint32_t m = 0;
+                        htpre.push_back((int32_t)(base + k));
+                        for (int lx = 0; lx < TS; lx++) {
+                            const int64_t p = (int64_t)(ty * TS + ly) * nx[g] + tx * TS + lx;
+                            const int32_t r = ci.cidx[p];
+                            if (r < 0) continue;
+                            m |= 1u << lx;
+                            C.cw[base + k] = ci.cw[r] & keep;
+                            C.pidx[base + k] = (uint16_t)(ci.pidx[r] + poff);
+                            C.rw[base + k] = 0u;
+                            C.outm[base + k] = 0;
+                            pos[k] = (int32_t)p;
+                            k++;
+                        }
+                        htmask.push_back(m);
+                    }
+                    if (k > k0) b->tiles_of[g].push_back(t);
+                }
+            if (k != n) return bail(set_error(MTS_ERR_STATE, "lattice " + std::to_string(g) + ": tile order lost cells"));
+            b->ntiles = t0 + ntx * nty;
+            b->cellPos[g] = pos;
+        }
+        for (int64_t c = 0; c < n && !b->tiled; c++) {
             const uint32_t cw = ci.cw[c];
             C.cw[base + c] = cw;
             C.pidx[base + c] = (uint16_t)(ci.pidx[c] + poff);
@@ -335,7 +478,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
             // lattice's first record
             C.rw[base + c] = (cw & CW_COMPLEX) ? (uint32_t)(ci.rw[c] + naux0) : ci.rw[c];
         }
-        for (int64_t k = 0; k < ci.naux; k++) {
+        for (int64_t k = 0; k < ci.naux && !b->tiled; k++) {
             const int32_t* r = &ci.aux[k * AUXW];
             for (int j = 0; j < 8; j++) C.aux.push_back(r[j] >= 0 ? (int32_t)(r[j] + base) : -1);
             for (int j = 8; j < AUXW; j++) C.aux.push_back(r[j]);
@@ -352,6 +495,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
         C.outm.resize(base, 0);
         if (g == 0) canonical_spec(C.spec);
     }
+    b->tfirst[nb] = b->ntiles;
     C.ncell = base;
     C.cstride = base;
     C.naux = (int64_t)C.aux.size() / AUXW;
@@ -378,13 +522,17 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
         o[2] = (float)q.s_d;
         o[3] = (float)q.r_srbb;
         o[4] = (float)(q.r_srbb - (double)o[3]);
+        o[5] = (float)q.r_mdbb;
     }
     const size_t nA = (size_t)base * 9;
     if ((rc = b->alloc(&cw, base)) || (rc = b->alloc(&rw, base)) || (rc = b->alloc(&pidx, base)) ||
         (rc = b->alloc(&outm, base)) || (rc = b->alloc(&aux, std::max<size_t>(C.aux.size(), 1))) ||
         (rc = b->alloc(&cpar, b->cpar_h.size())) || (rc = b->alloc(&b->A[0], nA)) || (rc = b->alloc(&b->A[1], nA)) ||
         (rc = b->alloc(&b->mac, (size_t)base * 3)) || (rc = b->alloc(&b->acc, 2 * (size_t)nb)) ||
-        (rc = b->alloc(&b->part, 2 * b->hblocks.size())) || (rc = b->alloc(&b->d_first, (size_t)nb + 1)) ||
+        (prm.wall == MTS_WALL_MDBB && ((rc = b->alloc(&b->M[0], (size_t)base * 4)) ||
+                                       (rc = b->alloc(&b->M[1], (size_t)base * 4)))) ||
+        (rc = b->alloc(&b->part, 2 * b->hblocks.size())) ||
+        (rc = b->alloc(&b->d_first, (size_t)nb + 1)) ||
         (rc = b->alloc(&b->d_blocks, b->hblocks.size())) || (rc = b->alloc(&b->d_queue, b->hblocks.size())) ||
         (rc = b->alloc(&b->d_nactive, 1)))
         return bail(rc);
@@ -408,6 +556,19 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->d_first, first.data(), (nb + 1) * 4, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b->part, 0, 2 * b->hblocks.size() * 8, b->stream);
+    if (e == hipSuccess && b->tiled) {
+        if ((rc = b->alloc(&b->d_tnb, htnb.size())) || (rc = b->alloc(&b->d_tmask, htmask.size())) ||
+            (rc = b->alloc(&b->d_tpre, htpre.size())) || (rc = b->alloc(&b->d_tqueue, (size_t)b->ntiles)) ||
+            (rc = b->alloc(&b->d_tfirst, (size_t)nb + 1)) || (rc = b->alloc(&b->d_ntactive, 1)) ||
+            (rc = b->alloc(&b->tpart, 2 * (size_t)b->ntiles)))
+            return bail(rc);
+        e = hipMemsetAsync(b->tpart, 0, 2 * (size_t)b->ntiles * 8, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(b->d_tnb, htnb.data(), htnb.size() * 4, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(b->d_tmask, htmask.data(), htmask.size() * 4, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(b->d_tpre, htpre.data(), htpre.size() * 4, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->d_tfirst, b->tfirst.data(), (nb + 1) * 4, hipMemcpyHostToDevice, b->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return bail(set_error(MTS_ERR_HIP, std::string("upload: ") + hipGetErrorString(e)));
     upload_spec(C.spec, b->stream);
@@ -510,8 +671,7 @@ int mts_batch_step(mts_batch b, int nsteps) {
     if (!b->tuned) {
         if (int rc = b->tune(8)) return rc;
     }
-    for (int i = 0; i < nsteps; i++)
-        if (int rc = b->step(false, 0)) return rc;
+    if (int rc = b->steps(nsteps)) return rc;
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(MTS_ERR_HIP, std::string("step: ") + hipGetErrorString(e));
     return MTS_OK;

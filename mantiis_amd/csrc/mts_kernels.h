@@ -24,6 +24,7 @@ struct CompactDev {
     const float* O = nullptr;  // interface overrides (multi-block buffer cells)
     int64_t OS = 0;
     int ovr = 0;  // 0 gather, 1 take the override, 2 skip (see CImg)
+    const float* M = nullptr;  // MDBB state read by the gather (set per launch)
 };
 
 struct LaunchCtx {
@@ -53,6 +54,10 @@ struct TransportSpec {
     double *rho = nullptr, *ux = nullptr, *uy = nullptr, *usq = nullptr;
     double* usq_old = nullptr;
     const double *min_rho = nullptr, *min_ux = nullptr, *min_uy = nullptr, *min_usq = nullptr;
+    // MDBB (FP32 cell order): the previous collided state of the wall cells
+    // read by the gather, and the one this pass writes (4 planes each)
+    const float* Min = nullptr;
+    float* Mout = nullptr;
 };
 
 struct ScSpec {
@@ -161,8 +166,37 @@ struct BatchLaunch {
     int from_rest = 0;
     int threads = 256;       // threads per 256-cell block: 64, 128 or 256
     int xrun = 32;           // XCD run length of the queue map (0: none)
+    const float* Min = nullptr;  // MDBB state (see TransportSpec)
+    float* Mout = nullptr;
 };
 int launch_cbatch(const BatchLaunch& L);
+// Batched tile kernels (mts_tile.inc): one block per TILE x TILE site tile of
+// a torus lattice in tile-major cell order, nsteps (1 or 2) steps per launch.
+constexpr int TILE = 32;
+struct TileLaunch {
+    hipStream_t stream = nullptr;
+    int coll = 0, wall = 0;
+    int nsteps = 2;
+    int grid = 0;                     // blocks (tiles past *nactive exit)
+    const float* Ain = nullptr;
+    float* Aout = nullptr;
+    const uint32_t* tw = nullptr;     // per cell: link bits | wall bits | type | CW_COMPLEX
+    const uint16_t* pidx = nullptr;
+    const void* cpar = nullptr;       // FP32 parameter records
+    const uint32_t* tmask = nullptr;  // per tile row: fluid mask
+    const int32_t* tpre = nullptr;    // per tile row: first cell
+    const int32_t* tnb = nullptr;     // per tile: 3 x 3 tiles around it
+    const int32_t* queue = nullptr;
+    const int32_t* nactive = nullptr;
+    float* mac = nullptr;
+    double* part = nullptr;
+    int64_t S = 0;
+    double rho0 = 0;
+    float Fbody = 0;
+    int xrun = 0;
+    bool one = false;
+};
+int launch_ctile(const TileLaunch& L);
 // acc[2g..2g+1] = sum of part over the blocks [first[g], first[g+1])
 void launch_bsum(const double* part, const int32_t* first, double* acc, int nb, hipStream_t st);
 // first substeps (fresh fields) of two levels in one launch

@@ -865,4 +865,5 @@ void launch_macro(const LaunchCtx& c, const double* fcan, double* rho, double* u
 #include "mts_phase.inc"
 #include "mts_fast.inc"
 #include "mts_compact.inc"
+#include "mts_tile.inc"
 #include "mts_multi.inc"

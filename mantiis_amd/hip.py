@@ -30,7 +30,7 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_destroy", "mts_cell_count", "mts_grid_info", "mts_links", "mts_set_operators",
            "mts_set_body_force", "mts_initialize", "mts_step_transport", "mts_step_adsorption",
            "mts_step_coupled", "mts_residual", "mts_method", "mts_field_size", "mts_get_field",
-           "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_get_stream", "mts_synchronize",
+           "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_permeability", "mts_get_stream", "mts_synchronize",
            "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows",
            "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy", "mts_probe_copy16", "mts_batch_create", "mts_batch_destroy",
            "mts_batch_cells", "mts_batch_run", "mts_batch_step", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
@@ -54,7 +54,8 @@ class MtsError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("dx", C.c_double), ("Fbody", C.c_double), ("T", C.c_double), ("P", C.c_double),
                 ("Gff", C.c_double), ("Gfs", C.c_double), ("collision", C.c_int), ("wall", C.c_int),
-                ("precision", C.c_int), ("multiblock", C.c_int), ("device", C.c_int), ("periodic_y", C.c_int)]
+                ("precision", C.c_int), ("multiblock", C.c_int), ("device", C.c_int), ("periodic_y", C.c_int),
+                ("batch_steps", C.c_int)]
 
 
 _LIB = None
@@ -96,6 +97,7 @@ def load_library(path=LIB_PATH):
     lib.mts_set_field.argtypes = [C.c_void_p, C.c_int, _D]
     lib.mts_complete_domain.argtypes = [C.c_void_p, _D, _D, _D]
     lib.mts_diagnostics.argtypes = [C.c_void_p, _D]
+    lib.mts_permeability.argtypes = [C.c_void_p, _D]
     lib.mts_get_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
     lib.mts_synchronize.argtypes = [C.c_void_p]
     lib.mts_grid_create.argtypes = [C.c_int, C.c_int, _I32, _D, _D, C.POINTER(Params), C.POINTER(C.c_void_p)]
@@ -247,6 +249,13 @@ class Lattice:
         out = np.zeros(4)
         _check(self.lib, self.lib.mts_diagnostics(self.h, _p(out, _D)))
         return out
+
+    def permeability(self):
+        """FP64 permeability report (mts_permeability) as a dict."""
+        out = np.zeros(8)
+        _check(self.lib, self.lib.mts_permeability(self.h, _p(out, _D)))
+        keys = ("darcy_velocity", "mean_ux", "flux", "mass", "porosity", "tau", "k_lattice", "k_m2")
+        return dict(zip(keys, out.tolist()))
 
     def grid(self):
         n = self.ncell

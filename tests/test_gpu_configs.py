@@ -1,0 +1,148 @@
+"""The benchmark configurations at their stated workloads (GPU).
+
+Shan-Chen adsorption (runtime fluid-solid strength G_ads in [-1.5, -0.5],
+initial density rho0 in [0.1, 0.5] with 1% seeded noise): the FP64 library
+step against the oracle with the same interaction strengths on a 128^2
+lattice, then FP32 (FP64 pseudopotential) against FP64 on the 2048^2 lattice
+of the benchmark.  The batched and quadtree configurations follow in their
+own tests below."""
+import sys
+import os
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from _gpu_util import banner, rel_l2
+from mantiis_amd.hip import Lattice
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402  (the benchmark's seeded input generators)
+
+W9 = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def shanchen_lattice(inputs, precision):
+    pore, kn, lp, G_ads, rho0, rho = inputs
+    lat = Lattice(pore, kn, lp, precision=precision, collision=1, wall=0, dx=1e-9, Fbody=0.0, Gfs=G_ads,
+                  periodic_y=1)
+    fl = pore.reshape(-1) == 1
+    r = rho.reshape(-1)[fl]
+    lat.set("f", (r[:, None] * W9[None, :]).reshape(-1))
+    lat.set("rho", r)
+    return lat
+
+
+@pytest.mark.parametrize("seed", [2027, 31])
+def test_shanchen_runtime_gads_fp64_against_oracle(seed):
+    inputs = bench.shanchen_inputs(n=128, seed=seed)
+    pore, kn, lp, G_ads, rho0, rho = inputs
+    banner(f"Shan-Chen 128^2 G_ads={G_ads:.4f} rho0={rho0:.4f}+1% noise, fp64 vs oracle, 100 steps")
+    assert -1.5 <= G_ads <= -0.5 and 0.1 <= rho0 <= 0.5
+    o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=0.0, periodic_y=True)
+    o.set_operators(1, 0)
+    o.set_interaction(-1.0, G_ads)
+    fl = pore.reshape(-1) == 1
+    r = rho.reshape(-1)[fl]
+    o.set("f", (r[:, None] * W9[None, :]).reshape(-1))
+    o.set("rho", r)
+    g = shanchen_lattice(inputs, "fp64")
+    o.run("adsorption", 100, 0.0)
+    g.step("adsorption", 100)
+    for name in ("f", "rho", "ux", "uy", "psi", "Fx", "Fy"):
+        a, b = g.get(name), o.get(name)
+        d = np.abs(a - b).max()
+        print(name, "max|gpu - oracle|", d, "scale", np.abs(b).max())
+        assert d <= 1e-15 * max(1.0, np.abs(b).max()), name
+    assert abs(g.residual() - o.diff_history()[-1]) <= 1e-12 * abs(o.diff_history()[-1])
+    g.close()
+    o.close()
+
+
+def test_shanchen_2048_fp32_against_fp64():
+    inputs = bench.shanchen_inputs()
+    pore, kn, lp, G_ads, rho0, rho = inputs
+    banner(f"Shan-Chen 2048^2 G_ads={G_ads:.4f} rho0={rho0:.4f}, fp32dev vs fp64, 200 steps")
+    a = shanchen_lattice(inputs, "fp64")
+    b = shanchen_lattice(inputs, "fp32dev")
+    m0 = a.diagnostics()[0]
+    a.step("adsorption", 200)
+    b.step("adsorption", 200)
+    da, db = a.diagnostics(), b.diagnostics()
+    print("mass drift fp64", (da[0] - m0) / m0, "fp32", (db[0] - m0) / m0)
+    assert abs(da[0] - m0) / m0 < 1e-12 and abs(db[0] - m0) / m0 < 1e-6
+    pa, pb = a.get("psi"), b.get("psi")
+    print("psi < 0 cells: fp64", int((pa < 0).sum()), "fp32", int((pb < 0).sum()))
+    assert int((pa < 0).sum()) == int((pb < 0).sum())
+    for name, tol in (("rho", 1e-6), ("psi", 1e-6), ("ux", 1e-3), ("uy", 1e-3)):
+        e = rel_l2(b.get(name), a.get(name))
+        print(name, "fp32 vs fp64 rel-l2", e)
+        assert e < tol, (name, e)
+    a.close()
+    b.close()
+
+
+def test_batched_config_fp32_against_fp64_and_oracle():
+    """The batched configuration as the benchmark builds it (256 lattices,
+    sides {128, 256, 512, 1024}, porosity U[0.4, 0.8], Kn log-uniform in
+    [0.01, 1]) run to steady state in one batch; the first lattice of each
+    side against the FP64 single-block path at the same step count
+    (permeability and velocity field); and two 128^2 lattices of the batch
+    against the oracle after 300 steps."""
+    from mantiis_amd.hip import Batch
+    banner("batched configuration: 256 lattices to steady state, FP64 and oracle checks")
+    geoms = bench.batched_inputs(256, workers=1)
+    b = Batch([(g[0], g[1], g[2]) for g in geoms], collision=2, wall=2, dx=1e-9, Fbody=1e-6)
+    r = b.run(400000, check_every=100, tol=1e-6)
+    print("converged", int((r["steps"] > 0).sum()), "max steps", int(np.abs(r["steps"]).max()), flush=True)
+    assert (r["steps"] > 0).all()
+    sizes = [g[0].shape[0] for g in geoms]
+    sample = [sizes.index(L) for L in (128, 256, 512, 1024) if L in sizes]
+    assert len(sample) == 4
+    res = bench.fp64_permeability_check(geoms, b, r["steps"], sample)
+    for x in res:
+        print(x, flush=True)
+        assert x["k_rel_err"] < 1e-4 and x["ux_rel_l2"] < 1e-3, x
+    b.close()
+    # two small lattices of the batch against the oracle
+    small = [i for i, L in enumerate(sizes) if L == 128][:2]
+    c = Batch([(geoms[i][0], geoms[i][1], geoms[i][2]) for i in small], collision=2, wall=2, dx=1e-9, Fbody=1e-6)
+    c.run(300, check_every=300, tol=1e-300)
+    for k, i in enumerate(small):
+        pore, kn, lp, knv = geoms[i]
+        o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-6, periodic_y=True)
+        o.run("transport", 300, 0.0)
+        oux, ouy, orho = o.complete_domain()
+        rho, ux, uy = c.fields(k)
+        e = dict(rho=rel_l2(rho.reshape(-1), orho), ux=rel_l2(ux.reshape(-1), oux), uy=rel_l2(uy.reshape(-1), ouy))
+        print("lattice", i, "Kn", knv, "batch FP32 vs oracle FP64 rel-l2", e, flush=True)
+        assert e["rho"] < 1e-6 and e["ux"] < 1e-3 and e["uy"] < 1e-2, e
+        o.close()
+    c.close()
+
+
+def test_quadtree_config_1024_fp32_against_fp64():
+    """The quadtree configuration of the benchmark (1024^2 disk packing,
+    porosity U[0.4, 0.8], Kn = 0.5, regularised MRT + SRBB, torus): the FP32
+    step run to steady state (mean-velocity change below 1e-6 per step),
+    then the FP64 library path for the same number of coarse steps; the
+    steady permeability (FP64 report) and the fields agree."""
+    banner("quadtree 1024^2: FP32 to steady state vs FP64 at the same step count")
+    pore, kn, lp, phi = bench.quadtree_inputs(1024)
+    a = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
+                multiblock=1)
+    steps, sec, crit, _ = bench.run_to_steady(a, 1e-6, 100, 100000, 120.0)
+    assert crit < 1e-6, crit
+    b = Lattice(pore, kn, lp, precision="fp64", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
+                multiblock=1)
+    b.step("transport", steps)
+    ka, kb = a.permeability(), b.permeability()
+    err = abs(ka["k_lattice"] - kb["k_lattice"]) / abs(kb["k_lattice"])
+    e = {k: rel_l2(a.get(k), b.get(k)) for k in ("rho", "ux", "uy")}
+    print("steps", steps, "k fp32", ka["k_lattice"], "k fp64", kb["k_lattice"], "rel err", err, "fields", e,
+          flush=True)
+    assert err < 1e-4 and e["rho"] < 1e-6 and e["ux"] < 1e-3
+    a.close()
+    b.close()

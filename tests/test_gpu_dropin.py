@@ -63,6 +63,27 @@ def test_app_reproduces_gold(kind, tmp_path):
     assert conv.size == 100
     assert os.path.exists(folder + "f.txt") and os.path.exists(folder + "SimulationDetails.txt")
     assert os.path.exists(folder + "ux_0.txt")  # dump at t = 0 (mantiisApp.hpp:328)
+    if kind == "transport":
+        # FP64 permeability report against the written fields and the oracle's
+        # relaxation times: U_D = sum u_x / N^2, k = nu U_D / F
+        rep = dict(line.split() for line in open(folder + "permeability.txt"))
+        rep = {k: float(v) for k, v in rep.items()}
+        ux = np.loadtxt(folder + "ux.txt")
+        rho = np.loadtxt(folder + "rho.txt")
+        pore, kn, lp = O.load_fixture_geometry()
+        o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-8)
+        tau = o.get("taus").mean()
+        o.close()
+        UD = ux.sum() / ux.size
+        print("permeability report", rep)
+        # the report is text at 15 significant digits
+        assert abs(rep["darcy_velocity"] - UD) <= 1e-12 * abs(UD)
+        assert abs(rep["flux"] - (rho * ux).sum()) <= 1e-12 * abs((rho * ux).sum())
+        assert abs(rep["porosity"] - (pore == 1).mean()) <= 1e-14
+        assert abs(rep["tau"] - tau) <= 1e-13 * tau
+        k = (tau - 0.5) / 3.0 * UD / 1e-8
+        assert abs(rep["k_lattice"] - k) <= 1e-12 * abs(k)
+        assert abs(rep["k_m2"] - k * 1e-18) <= 1e-12 * abs(k) * 1e-18
 
 
 def test_method_by_method_driver(tmp_path):
