@@ -1,34 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X D2Q9 step (one JSON line on stdout).
 
-Default workload (--workload quadtree): the quadtree multi-block transport
-case -- a 1024x1024 seeded disk packing with porosity drawn from
-U[0.4, 0.8], Kn = 0.5 at the finest level, regularised MRT collision with
-specular bounce-back, periodic in x and y, FP32 deviation storage.  A "step"
-is one coarse step of the quadtree: one fused substep of the coarse level,
-two of the fine level, and the interface update.  The run then continues
-from rest to steady state (mean-velocity change below 1e-6 per step) and
-reports the wall time.
+Default workload (--workload batched): the batched configuration -- 256
+seeded disk packings in one batch, side drawn from {128, 256, 512, 1024},
+porosity U[0.4, 0.8], disk radius U[3, 16], Kn log-uniform in [0.01, 1],
+regularised MRT collision with specular bounce-back, periodic in x and y,
+FP32 deviation storage.  A "step" is one launch of the batched transport
+kernel over every lattice of the batch (about 53 M fluid cells).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload quadtree|shanchen|transport|fp64|batched|warmstart|shape] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--repeats R]
+                    [--workload batched|quadtree|shanchen|transport|fp64|warmstart|shape]
+                    [--no-cpu] [--no-steady] [--no-fp64] [--eager]
 
-value = cell updates per second (MLUPS; a coarse step counts 2 N_1 + N_2) over
-exactly K steps timed with HIP events on the library's stream, inputs
-resident in HBM.  roofline: 72 B per cell update (nine FP32 populations read
-and written; metadata, stored fields and interface traffic not counted) over
-the coarse step's time, with the dominant launch reported beside it; for the
-single-block workloads, the fused pass's average launch time from HIP events
-recorded around it.  copy_peak: the measured copy ceiling of the same GPU.
-The CPU baseline is the oracle (the C++ restatement of the reference,
--Ofast -fopenmp) timed on a bounded sample.
+value = fluid-cell updates per second (MLUPS) over a window of exactly K
+steps, timed with HIP events on the batch's stream with the populations
+resident in HBM; R windows are timed and the median is reported.
+roofline: 72 B per cell update (nine FP32 populations read and written;
+metadata not counted) over the median window's time per launch, beside the
+measured copy ceiling of the same GPU; traffic: HBM bytes per launch from
+the committed rocprofv3 counter summary of the same workload.  Before the
+windows the batch runs every lattice from rest to steady state (the wall
+time to steady permeability), and sampled lattices are compared with the
+FP64 single-block path (permeability error).  The CPU baseline is the oracle
+(the C++ restatement of the reference, -Ofast -fopenmp) on one lattice of the
+batch, with the host's threads and with one.
 
-Other workloads: shanchen (Shan-Chen adsorption, 2048^2, 88 B per update),
-transport (single-block FP32 transport of a lattice of the quadtree
-workload's distribution, --n), fp64 (single-block FP64 transport, 256^2,
-144 B per update), batched (256 lattices of uneven size and Kn to steady
-state in one batch), warmstart (the learned warm start against plain runs),
-shape (Shape::calculateProperties, 1024^2).
+Other workloads: quadtree (multi-block transport of a 1024^2 packing, a step
+being one coarse step), shanchen (Shan-Chen adsorption, 2048^2, 88 B per
+update), transport (single-block FP32 transport, --n), fp64 (single-block
+FP64 transport, 256^2, 144 B per update), warmstart (the learned warm start
+against plain runs), shape (Shape::calculateProperties, 1024^2).
 """
 import argparse
 import json
@@ -148,7 +149,7 @@ def run_to_steady(lat, tol, chunk, max_steps, max_s):
     return n, time.perf_counter() - t0, crit, lat.residual()
 
 
-PROFILE_FOR = {"quadtree": "k_cmbx_quadtree_transport_1024.md", "shanchen": "k_csc_shanchen_adsorption_2048.md",
+PROFILE_FOR = {"batched": "k_cbat_batched_256.md", "quadtree": "k_cmbx_quadtree_transport_1024.md", "shanchen": "k_csc_shanchen_adsorption_2048.md",
                "transport": "k_ctr_transport_4096.md"}
 
 
@@ -249,66 +250,190 @@ def cpu_baseline(inputs, sample_steps):
                        f"serial as in the reference)")
 
 
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# The oracle's -Ofast build against the reference itself, measured in the
+# build container (Intel Xeon @ 2.70 GHz, 8 cores) on a 256^2 transport
+# lattice (RMRT + SRBB): the reference's 1.40 / 7.66 MLUPS at 1 / 8 threads
+# (stubbed-header build of the survey) against the oracle's 2.53 / 11.8.
+CPU_CALIBRATION = {"reference_mlups_1t": 1.40, "reference_mlups_8t": 7.66, "oracle_mlups_1t": 2.53,
+                   "oracle_mlups_8t": 11.83, "ratio_1t": 1.81, "ratio_8t": 1.54,
+                   "host": "build container, Intel(R) Xeon(R) Processor @ 2.70GHz, 8 cores",
+                   "note": "the oracle runs 1.5-1.8x faster than the reference on the same host, so this baseline "
+                           "overstates the reference's CPU speed"}
+
+
+def cpu_baseline_batched(geoms, g, steps_n, steps_1):
+    """Oracle (-Ofast -fopenmp) transport on lattice g of the batch: steps_n
+    steps with the host's OpenMP threads, steps_1 steps with one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    pore, kn, lp, knv = geoms[g]
+    lib = O.load(fast=True)
+    threads = int(lib.orc_threads())
+    out = {}
+    for th, st in ((threads, steps_n), (1, steps_1)):
+        lib.orc_set_threads(th)
+        o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-6, fast=True, periodic_y=True)
+        o.run("transport", 2, 0.0)  # warm-up
+        t0 = time.perf_counter()
+        o.run("transport", st, 0.0)
+        dt = time.perf_counter() - t0
+        out[th] = (o.ncell * st / dt / 1e6, o.ncell, dt)
+        o.close()
+    lib.orc_set_threads(threads)
+    n = pore.shape[0]
+    return dict(value=round(out[threads][0], 4), unit="MLUPS", cores=threads, kind="port",
+                one_thread=round(out[1][0], 4),
+                sample=f"lattice {g} of the batch ({n}x{n}, Kn {knv:.4g}, {out[threads][1]} fluid cells): "
+                       f"{steps_n} transport steps with {threads} OpenMP threads ({out[threads][2]:.1f} s), "
+                       f"{steps_1} with 1 thread ({out[1][2]:.1f} s); oracle built -Ofast -fopenmp",
+                host_cpu=host_cpu_model(), calibration=CPU_CALIBRATION)
+
+
+def fp64_permeability_check(geoms, b, steps, sample):
+    """Lattices `sample` of the batch against the FP64 single-block path of
+    the library run from rest for the same number of steps: relative error of
+    the permeability k = <u_x> nu / F (the FP32 batch's <u_x> against FP64's;
+    nu and F are the same) and of the velocity field."""
+    from mantiis_amd.hip import Lattice
+    out = []
+    for g in sample:
+        pore, kn, lp, knv = geoms[g]
+        st = int(abs(steps[g]))
+        lat = Lattice(pore, kn, lp, precision="fp64", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1)
+        lat.step("transport", st)
+        ux, uy, rho = lat.complete_domain()
+        lat.close()
+        brho, bux, buy = b.fields(g)
+        fl = pore.reshape(-1) == 1
+        k64 = float((rho[fl] * ux[fl]).sum() / rho[fl].sum())
+        k32 = float((brho.reshape(-1)[fl] * bux.reshape(-1)[fl]).sum() / brho.reshape(-1)[fl].sum())
+        du = float(np.linalg.norm(bux.reshape(-1)[fl] - ux[fl]) / np.linalg.norm(ux[fl]))
+        out.append({"lattice": int(g), "side": int(pore.shape[0]), "kn": round(knv, 5), "steps": st,
+                    "k_rel_err": abs(k32 - k64) / abs(k64), "ux_rel_l2": du})
+    return out
+
+
 def bench_batched(args):
-    """The batched configuration: every lattice from rest to steady state
-    (mean-velocity change below 1e-6 per step, checked every 100 steps) in
-    one batch; value = cell updates per second over that run."""
+    """The batched configuration (the default workload): 256 lattices of the
+    batched distribution in one batch.  (1) every lattice from rest to steady
+    state (mean-velocity change below 1e-6 per step, checked every 100 steps)
+    -- the wall time to steady permeability; (2) all lattices re-activated,
+    W warm-up steps, then `repeats` windows of exactly K steps, each one
+    k_cbat launch per step over the whole batch, timed with HIP events on the
+    batch's stream: value, ms_per_step and the roofline come from the median
+    window; (3) the FP32 permeability of sampled lattices against the FP64
+    single-block path; (4) the oracle on one lattice as the CPU baseline."""
     from mantiis_amd.hip import Batch
     from mantiis_amd.hiptime import StreamTimer
     t0 = time.perf_counter()
     geoms = batched_inputs(args.batch, workers=args.workers)
     t_gen = time.perf_counter() - t0
+    print(f"[bench] {len(geoms)} lattices generated in {t_gen:.1f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    b = Batch([(g[0], g[1], g[2]) for g in geoms], collision=2, wall=2, dx=1e-9, Fbody=1e-6)
+    b = Batch([(g[0], g[1], g[2]) for g in geoms], collision=2, wall=2, dx=1e-9, Fbody=1e-6,
+              batch_steps=args.batch_steps)
     t_setup = time.perf_counter() - t0
     cells = b.cells()
-    t = StreamTimer(b.stream_handle())
-    t.start()
-    r = b.run(400000, check_every=100, tol=1e-6, graph=not args.eager)
-    ms = t.stop()
-    t.close()
-    thr, xrun, tab = b.tune(-1)  # the configuration the run's autotuner chose
-    mlups = r["updates"] / (ms / 1e3) / 1e6
-    achieved = 72 * r["updates"] / (ms / 1e3) / 1e9
-    conv = r["steps"] > 0
-    sizes = np.array([g[0].shape[0] for g in geoms])
-    by_size = {int(L): {"lattices": int((sizes == L).sum()),
-                        "median_steps": int(np.median(np.abs(r["steps"][sizes == L]))) if (sizes == L).any() else 0,
-                        "max_steps": int(np.abs(r["steps"][sizes == L]).max()) if (sizes == L).any() else 0}
-               for L in (128, 256, 512, 1024)}
+    ncell = int(cells.sum())
+    print(f"[bench] batch of {ncell} cells set up in {t_setup:.1f} s", file=sys.stderr, flush=True)
+    thr, xrun, tab = b.tune(8)
+    spl, units = b.kernel()  # steps per launch of the step kernel (2: the tile kernels)
+    timer = StreamTimer(b.stream_handle())
+    steady = None
+    r = None
+    if not args.no_steady:
+        timer.start()
+        r = b.run(400000, check_every=100, tol=1e-6, graph=not args.eager)
+        run_ms = timer.stop()
+        conv = r["steps"] > 0
+        sizes = np.array([g[0].shape[0] for g in geoms])
+        by_size = {int(L): {"lattices": int((sizes == L).sum()),
+                            "median_steps": int(np.median(np.abs(r["steps"][sizes == L]))) if (sizes == L).any() else 0,
+                            "max_steps": int(np.abs(r["steps"][sizes == L]).max()) if (sizes == L).any() else 0}
+                   for L in (128, 256, 512, 1024)}
+        steady = {"criterion": "|d<u_x>/dt| / |<u_x>| < 1e-6 per step, checked every 100 steps, per lattice",
+                  "wall_s": round(run_ms / 1e3, 3), "converged": int(conv.sum()),
+                  "max_steps": int(np.abs(r["steps"]).max()), "cell_updates": int(r["updates"]),
+                  "mlups_over_run": round(r["updates"] / (run_ms / 1e3) / 1e6, 2), "graph": not args.eager,
+                  "by_side": by_size}
+        print(f"[bench] steady state of all lattices in {run_ms / 1e3:.2f} s", file=sys.stderr, flush=True)
+    # the timed windows: every lattice active
+    b.activate(np.ones(len(geoms), np.int32))
+    if args.warmup > 0:
+        b.step(args.warmup)
+    reps = []
+    for _ in range(max(args.repeats, 1)):
+        timer.start()
+        b.step(args.steps)
+        reps.append(timer.stop())
+    timer.close()
+    ms = float(np.median(reps))
+    ms_step = ms / args.steps
+    mlups = ncell * args.steps / (ms / 1e3) / 1e6
+    achieved = 72 * ncell / (ms_step / 1e3) / 1e9
+    print(f"[bench] {args.repeats} windows of {args.steps} steps: median {ms_step:.4f} ms per step",
+          file=sys.stderr, flush=True)
+    fp64 = None
+    if r is not None and not args.no_fp64:
+        sizes = [g[0].shape[0] for g in geoms]
+        sample = [sizes.index(L) for L in (128, 256, 512, 1024) if L in sizes]
+        fp64 = fp64_permeability_check(geoms, b, r["steps"], sample)
+        print("[bench] FP32 vs FP64 permeability checked", file=sys.stderr, flush=True)
     b.close()
     copy = copy_peak_gbs()
+    cpu = None
+    if not args.no_cpu:
+        sizes = [g[0].shape[0] for g in geoms]
+        g512 = sizes.index(512) if 512 in sizes else 0
+        cpu = cpu_baseline_batched(geoms, g512, args.cpu_steps or 600, max((args.cpu_steps or 600) // 10, 10))
+    traffic = profiled_traffic("batched", 0)
     out = {
         "metric": "MLUPS & %HBM roofline",
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
-        "steps": int(np.abs(r["steps"]).max()),
-        "warmup": 0,
-        "ms_per_step": None,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 6),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 (populations)",
         "data": "synthetic (seeded disk packings)",
         "config": {"workload": f"batched: {args.batch} lattices, side U{{128,256,512,1024}}, porosity U[0.4,0.8], "
-                               "disk radius U[3,16], Kn log-uniform [0.01,1], RMRT+SRBB, torus, each run from rest "
-                               "to |d<u_x>/dt|/|<u_x>| < 1e-6 per step (checked every 100 steps) in one batch",
-                   "lattices": args.batch, "fluid_cells": int(cells.sum()), "parallelism": "single GPU",
-                   "by_side": by_size},
-        "roofline": {"bound": "hbm", "kernel": "k_cbat (one launch per step over the queue of unconverged lattices; "
-                                               "whole run incl. convergence checks)",
+                               "disk radius U[3,16], Kn log-uniform [0.01,1], RMRT+SRBB, torus, FP32 deviations; "
+                               "a step is one k_cbat launch over every lattice of the batch",
+                   "lattices": args.batch, "fluid_cells": ncell, "parallelism": "single GPU",
+                   "repeats": len(reps), "window_ms": [round(v, 4) for v in reps]},
+        "roofline": {"bound": "hbm",
+                     "kernel": ("k_ctile<RMRT,SRBB,2> (batched FP32 transport: a 32x16-site tile and its two-site "
+                                "ring staged in LDS, two steps per launch over the tile queue)") if spl == 2 else
+                               ("k_cbat<RMRT,SRBB> (batched FP32 transport: gather-stream + SRBB + RMRT, one step "
+                                "per launch over the queue of 256-cell blocks)"),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_cell_update": 72,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_cell_update": 72,
+                     "steps_per_launch": spl, "cells_per_launch": ncell,
+                     "algorithmic_bytes_per_launch": 72 * ncell * spl,
+                     "kernel_ms": round(ms_step * spl, 6), "launches_timed": args.steps // spl * len(reps),
                      "copy_peak": round(copy, 1), "frac_of_copy": round(achieved / copy, 4)},
-        "cpu_baseline": None,
-        "batch": {"converged": int(conv.sum()), "run_s": round(ms / 1e3, 3), "cell_updates": int(r["updates"]),
-                  "graph": not args.eager,
-                  "generation_s": round(t_gen, 1), "setup_s": round(t_setup, 1)},
+        "cpu_baseline": cpu,
+        "steady": steady,
+        "fp32_vs_fp64": fp64,
         "autotune": {"threads_per_block": thr, "xcd_run": xrun,
                      "ms_per_step": [[round(v, 4) for v in row] for row in tab.tolist()],
                      "candidates": "rows: threads per 256-cell block 64, 128, 256; columns: XCD run 0, 8, 32; "
-                                   "timed inside the run (8 steps each from its first state, state restored)"},
+                                   "8 steps each from the first state, state restored"},
+        "batch_setup": {"generation_s": round(t_gen, 1), "setup_s": round(t_setup, 1)},
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -322,8 +447,6 @@ def bench_warmstart(args):
     warm-started wall time of the batched solve."""
     from mantiis_amd import warmstart as WS
     from mantiis_amd.hip import globals_for
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import unet_oracle as U
     F = 1e-6
     rho_in = globals_for(dx=1e-9)[1]["rhoin"]
     t0 = time.perf_counter()
@@ -337,10 +460,7 @@ def bench_warmstart(args):
     net, hist = WS.train(ds, c0=c0, batch=B, epochs=args.epochs, lr=1e-3)
     t_train = time.perf_counter() - t0
     steps_train = args.epochs * (ds.n // B)
-    flops = 0
-    lv = [0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0]
-    for i, (cin, cout, k, relu) in enumerate(U.layer_table(c0)):
-        flops += 2 * B * (256 >> lv[i]) ** 2 * cin * cout * k * k
+    flops = net.flops(B, 256, 256)
     tflops = 3 * flops * steps_train / t_train / 1e12  # forward + two backward products per layer
     params = net.params()
     net.close()
@@ -462,9 +582,14 @@ def bench_shape(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="quadtree", choices=["quadtree", "shanchen", "batched", "warmstart", "transport", "fp64", "shape"])
+    ap.add_argument("--repeats", type=int, default=5, help="batched: timed windows of --steps steps (median reported)")
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "quadtree", "shanchen", "warmstart", "transport", "fp64", "shape"])
+    ap.add_argument("--no-fp64", action="store_true", help="batched: skip the FP64 permeability check")
+    ap.add_argument("--batch-steps", type=int, default=1, choices=[1, 2],
+                    help="batched: steps per launch (1: one-step cell-order kernel, 2: tile kernels)")
     ap.add_argument("--train", type=int, default=2000, help="warmstart: training lattices (256^2)")
     ap.add_argument("--eval", type=int, default=64, help="warmstart: unseen evaluation lattices (512^2)")
     ap.add_argument("--epochs", type=int, default=20, help="warmstart: training epochs")
@@ -473,7 +598,7 @@ def main():
     ap.add_argument("--eager", action="store_true",
                     help="batched: launch the steps one by one instead of replaying a captured graph per chunk "
                          "(rocprofv3's kernel trace fails inside HIP graph instantiation)")
-    ap.add_argument("--no-steady", action="store_true", help="quadtree: skip the run to steady state")
+    ap.add_argument("--no-steady", action="store_true", help="batched / quadtree: skip the run to steady state")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU sample steps (default: quadtree 1000, shanchen 16)")
     ap.add_argument("--cpu-n", type=int, default=256, help="quadtree: lattice side of the CPU sample")

@@ -75,10 +75,10 @@ typedef struct mts_params {
                       * 1: torus -- links across either seam are classified
                       *    like interior links (mass-conserving)            */
     int batch_steps; /* batched solver (mts_batch_*): time steps per launch --
+                      * 1: one step per launch in cell order (default);
                       * 2: the tile kernels, two steps per launch staged in
-                      *    LDS (default; every side must be a multiple of 32,
-                      *    else the batch runs one step per launch);
-                      * 1: one step per launch in cell order               */
+                      *    LDS (every side must be a multiple of 32 x 16,
+                      *    else one step per launch; not with MDBB walls)  */
 } mts_params;
 
 /* Fills the defaults above. */
@@ -283,6 +283,10 @@ int mts_batch_run(mts_batch b, int64_t max_steps, int check_every, double tol, i
  * stream; mts_batch_fields reports the state of the last statistics step
  * (creation or the last check of mts_batch_run). */
 int mts_batch_step(mts_batch b, int nsteps);
+/* the batch's step kernel: *steps_per_launch = 2 for the tile kernels, 1 for
+ * the one-step cell-order kernel; *units = its tiles (32 x 32 sites) or
+ * 256-cell blocks (either pointer may be NULL) */
+int mts_batch_kernel(mts_batch b, int* steps_per_launch, int64_t* units);
 /* marks lattices active (1) or not (0) for the next runs */
 int mts_batch_activate(mts_batch b, const int32_t* active);
 /* rho, u_x, u_y (nx*ny row-major, solids 0) of lattice g's post-stream
@@ -321,6 +325,10 @@ typedef struct mts_net_s* mts_net;
 int mts_net_create(int c0, int max_batch, int max_h, int max_w, uint64_t seed, mts_net* out);
 int mts_net_destroy(mts_net n);
 int mts_net_param_count(mts_net n, int64_t* count);
+/* floating-point operations of one forward pass over B images of H x W
+ * (2 per multiply-add of every convolution; a training step's backward
+ * products are twice that) */
+int mts_net_flops(mts_net n, int B, int H, int W, double* fwd);
 /* parameters and gradients as one FP32 host array: per layer the weights
  * [3*3 or 1][cout][cin] then the biases [cout] */
 int mts_net_get_params(mts_net n, float* host);

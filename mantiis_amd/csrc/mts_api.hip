@@ -69,6 +69,7 @@ struct mts_sim_s {
     // FP32 carried populations live in cell order (single block)
     CompactImage cimg_h;
     std::vector<float> cpar_h;
+    const float* d_book = nullptr;  // SRBB bookkeeping table of the FP32 parameter records
     CompactDev cimg;
     bool use_cimg = false;
     double* d_f = nullptr;
@@ -263,22 +264,16 @@ static int upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
 
 // FP32 parameter records (s_s, s_q, s_d, r as hi + lo) of a parameter table.
 static int upload_cpar(mts_sim_s* s, const std::vector<CellParam>& params, const float** out) {
-    s->cpar_h.assign(params.size() * 8, 0.0f);
-    for (size_t i = 0; i < params.size(); i++) {
-        const CellParam& q = params[i];
-        float* o = &s->cpar_h[i * 8];
-        o[0] = (float)q.s_s;
-        o[1] = (float)q.s_q;
-        o[2] = (float)q.s_d;
-        o[3] = (float)q.r_srbb;
-        o[4] = (float)(q.r_srbb - (double)o[3]);
-        o[5] = (float)q.r_mdbb;
-    }
+    std::vector<float> book;
+    fp32_params(params, s->ph.rhoin, s->cpar_h, book);
     float* cpar;
+    float* dbook;
     int rc;
     if ((rc = s->alloc(&cpar, s->cpar_h.size())) ||
-        (rc = upload(cpar, s->cpar_h.data(), s->cpar_h.size() * 4, s->stream)))
+        (rc = upload(cpar, s->cpar_h.data(), s->cpar_h.size() * 4, s->stream)) ||
+        (rc = s->alloc(&dbook, book.size())) || (rc = upload(dbook, book.data(), book.size() * 4, s->stream)))
         return rc;
+    s->d_book = dbook;
     *out = cpar;
     return MTS_OK;
 }
@@ -313,6 +308,7 @@ static int upload_compact(mts_sim_s* s, const CompactImage& h, const float* cpar
     d.segs = segs;
     d.cellPos = cpos;
     d.cpar = cpar;
+    d.book = s->d_book;
     d.ncell = h.ncell;
     d.cstride = h.cstride;
     d.ntx = h.ntx;
@@ -982,7 +978,7 @@ void mts_default_params(mts_params* p) {
     p->multiblock = 0;
     p->device = 0;
     p->periodic_y = 0;
-    p->batch_steps = 2;
+    p->batch_steps = 1;
 }
 
 const char* mts_last_error(void) { return g_err.c_str(); }

@@ -42,6 +42,7 @@ struct mts_batch_s {
     std::vector<CellParam> params;
     CompactImage cat;                          // concatenated image (host)
     std::vector<float> cpar_h;
+    float* book = nullptr;  // SRBB bookkeeping table of the FP32 parameter records
     CompactDev dev;
     std::vector<BatBlock> hblocks;
     std::vector<std::vector<int32_t>> blocks_of;  // per lattice: its block ids
@@ -173,6 +174,7 @@ struct mts_batch_s {
         T.tw = dev.cw;
         T.pidx = dev.pidx;
         T.cpar = dev.cpar;
+        T.book = dev.book;
         T.tmask = d_tmask;
         T.tpre = d_tpre;
         T.tnb = d_tnb;
@@ -400,7 +402,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     // tile kernels when asked for and every side is a multiple of the tile
     b->tiled = prm.batch_steps == 2 && prm.wall != MTS_WALL_MDBB;  // the tile kernels have no MDBB
     for (int g = 0; g < nb && b->tiled; g++)
-        if (nx[g] % TILE || ny[g] % TILE) b->tiled = false;
+        if (nx[g] % TILE_X || ny[g] % TILE_Y) b->tiled = false;
     std::vector<int32_t> htnb, htpre;
     std::vector<uint32_t> htmask;
     CompactImage& C = b->cat;
@@ -433,8 +435,8 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
             // fluid sites in row-major order; per cell the link bits, the
             // boundary type and whether the one-step pass applies the wall
             // rule in link order
-            constexpr int TS = TILE;
-            const int ntx = nx[g] / TS, nty = ny[g] / TS, t0 = b->ntiles;
+            constexpr int TX = TILE_X, TY = TILE_Y;
+            const int ntx = nx[g] / TX, nty = ny[g] / TY, t0 = b->ntiles;
             b->tfirst[g] = t0;
             std::vector<int32_t> pos(n);
             int64_t k = 0;
@@ -446,11 +448,11 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
                         for (int dx = -1; dx <= 1; dx++)
                             htnb.push_back(t0 + ((ty + dy + nty) % nty) * ntx + (tx + dx + ntx) % ntx);
                     const int64_t k0 = k;
-                    for (int ly = 0; ly < TS; ly++) {
+                    for (int ly = 0; ly < TY; ly++) {
                         uint32_t m = 0;
                         htpre.push_back((int32_t)(base + k));
-                        for (int lx = 0; lx < TS; lx++) {
-                            const int64_t p = (int64_t)(ty * TS + ly) * nx[g] + tx * TS + lx;
+                        for (int lx = 0; lx < TX; lx++) {
+                            const int64_t p = (int64_t)(ty * TY + ly) * nx[g] + tx * TX + lx;
                             const int32_t r = ci.cidx[p];
                             if (r < 0) continue;
                             m |= 1u << lx;
@@ -513,21 +515,13 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     uint16_t* pidx;
     uint8_t* outm;
     float* cpar;
-    b->cpar_h.assign(b->params.size() * 8, 0.0f);
-    for (size_t i = 0; i < b->params.size(); i++) {
-        const CellParam& q = b->params[i];
-        float* o = &b->cpar_h[i * 8];
-        o[0] = (float)q.s_s;
-        o[1] = (float)q.s_q;
-        o[2] = (float)q.s_d;
-        o[3] = (float)q.r_srbb;
-        o[4] = (float)(q.r_srbb - (double)o[3]);
-        o[5] = (float)q.r_mdbb;
-    }
+    std::vector<float> book;
+    fp32_params(b->params, b->ph.rhoin, b->cpar_h, book);
     const size_t nA = (size_t)base * 9;
     if ((rc = b->alloc(&cw, base)) || (rc = b->alloc(&rw, base)) || (rc = b->alloc(&pidx, base)) ||
         (rc = b->alloc(&outm, base)) || (rc = b->alloc(&aux, std::max<size_t>(C.aux.size(), 1))) ||
-        (rc = b->alloc(&cpar, b->cpar_h.size())) || (rc = b->alloc(&b->A[0], nA)) || (rc = b->alloc(&b->A[1], nA)) ||
+        (rc = b->alloc(&cpar, b->cpar_h.size())) || (rc = b->alloc(&b->book, book.size())) ||
+        (rc = b->alloc(&b->A[0], nA)) || (rc = b->alloc(&b->A[1], nA)) ||
         (rc = b->alloc(&b->mac, (size_t)base * 3)) || (rc = b->alloc(&b->acc, 2 * (size_t)nb)) ||
         (prm.wall == MTS_WALL_MDBB && ((rc = b->alloc(&b->M[0], (size_t)base * 4)) ||
                                        (rc = b->alloc(&b->M[1], (size_t)base * 4)))) ||
@@ -544,6 +538,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
         e = hipMemcpyAsync(aux, C.aux.data(), C.aux.size() * 4, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(cpar, b->cpar_h.data(), b->cpar_h.size() * 4, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->book, book.data(), book.size() * 4, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->d_blocks, b->hblocks.data(), b->hblocks.size() * sizeof(BatBlock), hipMemcpyHostToDevice,
                            b->stream);
@@ -579,6 +574,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     b->dev.outm = outm;
     b->dev.aux = aux;
     b->dev.cpar = cpar;
+    b->dev.book = b->book;
     b->dev.ncell = base;
     b->dev.cstride = base;
     // drop the host copies of what now lives on the device
@@ -662,6 +658,13 @@ int mts_batch_run(mts_batch b, int64_t max_steps, int check_every, double tol, i
         if (ux_mean) ux_mean[g] = b->active[g] || st[g] > 0 ? um[g] : b->u_prev[g];
     }
     if (cell_updates) *cell_updates = upd;
+    return MTS_OK;
+}
+
+int mts_batch_kernel(mts_batch b, int* steps_per_launch, int64_t* units) {
+    if (int rc = check(b)) return rc;
+    if (steps_per_launch) *steps_per_launch = b->tiled ? 2 : 1;
+    if (units) *units = b->tiled ? (int64_t)b->ntiles : (int64_t)b->hblocks.size();
     return MTS_OK;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "mts_common.h"
 
@@ -18,6 +19,7 @@ struct CompactDev {
     const int32_t* cidx = nullptr;
     const int32_t* segs = nullptr;
     const void* cpar = nullptr;  // FP32 parameter records (32 B per (tau_s, Kn) pair)
+    const void* book = nullptr;  // SRBB bookkeeping table (24 floats per pair)
     const int32_t* cellPos = nullptr;
     int64_t ncell = 0, cstride = 0;
     int ntx = 0;
@@ -143,6 +145,9 @@ int launch_cinterface_step(const LaunchCtx& c1, const MbSpec& s1, const LaunchCt
 void upload_icsr(hipStream_t st);
 // FP32 production kernels in cell order (mts_compact.inc)
 void upload_spec(const uint32_t* spec64, hipStream_t st);
+// The FP32 parameter records (8 floats per (tau_s, Kn) pair) and the SRBB
+// bookkeeping table (24 floats per pair) of a parameter list.
+void fp32_params(const std::vector<CellParam>& params, double rho0, std::vector<float>& cpar, std::vector<float>& book);
 int launch_ctransport(const LaunchCtx& c, int mode, const TransportSpec& s);
 int launch_cshanchen(const LaunchCtx& c, int kind, int mode, const ScSpec& s);
 // mode 0 from the canonical state, 1 gather; mac 0 fresh fields, 1 stored
@@ -170,9 +175,10 @@ struct BatchLaunch {
     float* Mout = nullptr;
 };
 int launch_cbatch(const BatchLaunch& L);
-// Batched tile kernels (mts_tile.inc): one block per TILE x TILE site tile of
-// a torus lattice in tile-major cell order, nsteps (1 or 2) steps per launch.
-constexpr int TILE = 32;
+// Batched tile kernels (mts_tile.inc): one block per TILE_X x TILE_Y site
+// tile of a torus lattice in tile-major cell order, nsteps (1 or 2) steps per
+// launch.
+constexpr int TILE_X = 32, TILE_Y = 16;
 struct TileLaunch {
     hipStream_t stream = nullptr;
     int coll = 0, wall = 0;
@@ -183,6 +189,7 @@ struct TileLaunch {
     const uint32_t* tw = nullptr;     // per cell: link bits | wall bits | type | CW_COMPLEX
     const uint16_t* pidx = nullptr;
     const void* cpar = nullptr;       // FP32 parameter records
+    const void* book = nullptr;       // SRBB bookkeeping table
     const uint32_t* tmask = nullptr;  // per tile row: fluid mask
     const int32_t* tpre = nullptr;    // per tile row: first cell
     const int32_t* tnb = nullptr;     // per tile: 3 x 3 tiles around it

@@ -643,6 +643,21 @@ int mts_net_destroy(mts_net n) {
     return MTS_OK;
 }
 
+int mts_net_flops(mts_net n, int B, int H, int W, double* fwd) {
+    if (int rc = check_net(n)) return rc;
+    if (!fwd || B <= 0 || H <= 0 || W <= 0) return set_error(MTS_ERR_ARG, "bad argument");
+    // resolution level of each layer's input: two convolutions per encoder
+    // level 0..3, two per decoder level 2..0, then the 1x1 head at level 0
+    static const int level[15] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+    double f = 0.0;
+    for (size_t i = 0; i < n->L.size() && i < 15; i++) {
+        const ConvL& l = n->L[i];
+        f += 2.0 * B * (double)(H >> level[i]) * (double)(W >> level[i]) * l.cin * l.cout * l.ksz * l.ksz;
+    }
+    *fwd = f;
+    return MTS_OK;
+}
+
 int mts_net_param_count(mts_net n, int64_t* count) {
     if (int rc = check_net(n)) return rc;
     if (!count) return set_error(MTS_ERR_ARG, "null argument");

@@ -33,9 +33,9 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_permeability", "mts_get_stream", "mts_synchronize",
            "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows",
            "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy", "mts_probe_copy16", "mts_batch_create", "mts_batch_destroy",
-           "mts_batch_cells", "mts_batch_run", "mts_batch_step", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
+           "mts_batch_cells", "mts_batch_run", "mts_batch_step", "mts_batch_kernel", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
            "mts_batch_stream", "mts_batch_tune", "mts_batch_set_config", "mts_dev_alloc", "mts_dev_free", "mts_dev_upload", "mts_dev_download",
-           "mts_net_create", "mts_net_destroy", "mts_net_param_count", "mts_net_get_params", "mts_net_set_params",
+           "mts_net_create", "mts_net_destroy", "mts_net_param_count", "mts_net_flops", "mts_net_get_params", "mts_net_set_params",
            "mts_net_get_grads", "mts_net_forward", "mts_net_backward", "mts_net_adam", "mts_net_gemm_check",
            "mts_net_stream", "mts_shape_create", "mts_shape_destroy", "mts_shape_stream", "mts_shape_run", "mts_shape_run_static",
            "mts_shape_properties"]
@@ -115,6 +115,7 @@ def load_library(path=LIB_PATH):
     lib.mts_batch_cells.argtypes = [C.c_void_p, _I64]
     lib.mts_batch_run.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_int, _I64, _D, _D, _I64]
     lib.mts_batch_step.argtypes = [C.c_void_p, C.c_int]
+    lib.mts_batch_kernel.argtypes = [C.c_void_p, C.POINTER(C.c_int), _I64]
     lib.mts_batch_activate.argtypes = [C.c_void_p, _I32]
     lib.mts_batch_fields.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
     lib.mts_batch_set_state.argtypes = [C.c_void_p, C.c_int, _D]
@@ -408,6 +409,13 @@ class Batch:
                                                 _p(steps, _I64), _p(sec, _D), _p(ux, _D), C.byref(upd)))
         return dict(steps=steps, seconds=sec, ux=ux, updates=upd.value)
 
+    def kernel(self):
+        """(steps per launch, tiles or 256-cell blocks) of the batch's step kernel."""
+        k = C.c_int(0)
+        u = C.c_int64(0)
+        _check(self.lib, self.lib.mts_batch_kernel(self.h, C.byref(k), C.byref(u)))
+        return k.value, u.value
+
     def step(self, nsteps):
         """nsteps steps of the active lattices (mts_batch_step): no statistics,
         no convergence test; asynchronous on the batch's stream."""
@@ -498,6 +506,7 @@ class Net:
         self.lib = load_library()
         for name, args in (("mts_net_create", [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
                            ("mts_net_destroy", [C.c_void_p]), ("mts_net_param_count", [C.c_void_p, _I64]),
+                           ("mts_net_flops", [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
                            ("mts_net_get_params", [C.c_void_p, C.POINTER(C.c_float)]),
                            ("mts_net_set_params", [C.c_void_p, C.POINTER(C.c_float)]),
                            ("mts_net_get_grads", [C.c_void_p, C.POINTER(C.c_float)]),
@@ -525,6 +534,13 @@ class Net:
             self.close()
         except Exception:
             pass
+
+    def flops(self, batch, h, w):
+        """Floating-point operations of one forward pass over `batch` images
+        of h x w (2 per multiply-add of every convolution)."""
+        f = C.c_double()
+        _check(self.lib, self.lib.mts_net_flops(self.h, batch, h, w, C.byref(f)))
+        return f.value
 
     def params(self):
         out = np.zeros(self.nparam, np.float32)

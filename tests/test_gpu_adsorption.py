@@ -65,7 +65,8 @@ def test_adsorption_fp32_deviation(fixture_geometry):
     oux, ouy, orho = o.complete_domain()
     e = dict(ux=rel_l2(ux, oux), uy=rel_l2(uy, ouy), rho=rel_l2(rho, orho))
     print("FP32-dev rel-l2 vs FP64 oracle", e)
-    assert e["rho"] < 1e-5 and e["ux"] < 2e-3 and e["uy"] < 2e-3
+    # measured: rho 1.6e-7, ux 1.1e-6, uy 1.0e-6
+    assert e["rho"] < 2e-6 and e["ux"] < 2e-5 and e["uy"] < 2e-5
 
 
 def test_shanchen_edge_tiles_fp64_and_fp32():
@@ -90,7 +91,7 @@ def test_shanchen_edge_tiles_fp64_and_fp32():
         e = rel_l2(b.get(name), a.get(name))
         print(name, "fp64 vs oracle", d, "fp32 vs fp64", e)
         assert d <= 1e-15 * max(1.0, np.abs(o.get(name)).max()), name
-        assert e < 2e-3, name
+        assert e < 1e-5, name  # measured: at most 5.8e-7 (ux)
 
 
 def test_non_square_lattice():
@@ -110,7 +111,7 @@ def test_non_square_lattice():
     print("mass", (da[0] - m0) / m0, (db[0] - m0) / m0, "rel ux", rel_l2(b.get("ux"), a.get("ux")))
     assert np.isfinite(da).all() and np.isfinite(db).all()
     assert abs(da[0] - m0) / m0 < 1e-12 and abs(db[0] - m0) / m0 < 1e-6
-    assert rel_l2(b.get("ux"), a.get("ux")) < 1e-3
+    assert rel_l2(b.get("ux"), a.get("ux")) < 1e-5  # measured: 6.6e-7
 
 
 def test_shanchen_fp32_across_tile_columns():

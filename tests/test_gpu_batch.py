@@ -103,8 +103,8 @@ def test_batch_warm_start_from_converged_state():
 
 @pytest.mark.parametrize("odd", [False, True])
 def test_tiled_batch_equals_one_step_batch(odd):
-    """The tile kernels (two steps per launch, staged in LDS; the default for
-    sides that are multiples of 32) against the one-step cell-order kernel:
+    """The tile kernels (two steps per launch, staged in LDS; batch_steps = 2,
+    sides multiples of 32 x 16) against the one-step cell-order kernel:
     the same fields after the same steps to FP32 rounding (the same wall rule
     and collision functions, contracted independently by the compiler) -- an
     even count (two-step launches only) and an odd one (a one-step tile

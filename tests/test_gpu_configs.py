@@ -76,7 +76,7 @@ def test_shanchen_2048_fp32_against_fp64():
     pa, pb = a.get("psi"), b.get("psi")
     print("psi < 0 cells: fp64", int((pa < 0).sum()), "fp32", int((pb < 0).sum()))
     assert int((pa < 0).sum()) == int((pb < 0).sum())
-    for name, tol in (("rho", 1e-6), ("psi", 1e-6), ("ux", 1e-3), ("uy", 1e-3)):
+    for name, tol in (("rho", 2e-7), ("psi", 1e-7), ("ux", 2e-5), ("uy", 2e-5)):
         e = rel_l2(b.get(name), a.get(name))
         print(name, "fp32 vs fp64 rel-l2", e)
         assert e < tol, (name, e)
@@ -104,7 +104,8 @@ def test_batched_config_fp32_against_fp64_and_oracle():
     res = bench.fp64_permeability_check(geoms, b, r["steps"], sample)
     for x in res:
         print(x, flush=True)
-        assert x["k_rel_err"] < 1e-4 and x["ux_rel_l2"] < 1e-3, x
+        # measured: k at most 7.1e-6, ux at most 1.03e-5 (512^2, Kn 0.68)
+        assert x["k_rel_err"] < 1e-4 and x["ux_rel_l2"] < 1e-4, x
     b.close()
     # two small lattices of the batch against the oracle
     small = [i for i, L in enumerate(sizes) if L == 128][:2]
@@ -118,7 +119,8 @@ def test_batched_config_fp32_against_fp64_and_oracle():
         rho, ux, uy = c.fields(k)
         e = dict(rho=rel_l2(rho.reshape(-1), orho), ux=rel_l2(ux.reshape(-1), oux), uy=rel_l2(uy.reshape(-1), ouy))
         print("lattice", i, "Kn", knv, "batch FP32 vs oracle FP64 rel-l2", e, flush=True)
-        assert e["rho"] < 1e-6 and e["ux"] < 1e-3 and e["uy"] < 1e-2, e
+        # measured: rho 2.3e-10, ux 6.4e-7, uy 9.9e-7
+        assert e["rho"] < 1e-8 and e["ux"] < 1e-5 and e["uy"] < 1e-5, e
         o.close()
     c.close()
 
@@ -143,6 +145,7 @@ def test_quadtree_config_1024_fp32_against_fp64():
     e = {k: rel_l2(a.get(k), b.get(k)) for k in ("rho", "ux", "uy")}
     print("steps", steps, "k fp32", ka["k_lattice"], "k fp64", kb["k_lattice"], "rel err", err, "fields", e,
           flush=True)
-    assert err < 1e-4 and e["rho"] < 1e-6 and e["ux"] < 1e-3
+    # measured: k 6.2e-7, rho 5.2e-10, ux 1.9e-6, uy 2.1e-6
+    assert err < 1e-5 and e["rho"] < 1e-8 and e["ux"] < 2e-5 and e["uy"] < 2e-5
     a.close()
     b.close()

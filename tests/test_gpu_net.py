@@ -38,6 +38,11 @@ def test_forward_backward_match_oracle():
     c0, B, H, W = 8, 2, 32, 32
     net = Net(c0=c0, max_batch=B, max_h=H, max_w=W, seed=3)
     assert net.nparam == U.param_count(c0)
+    # two flops per multiply-add; the encoder halves the sides per level
+    level = [0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0]
+    want = sum(2 * B * (H >> lv) * (W >> lv) * cin * cout * k * k
+               for lv, (cin, cout, k, _) in zip(level, U.layer_table(c0)))
+    assert net.flops(B, H, W) == want
     rng = np.random.default_rng(5)
     x = rng.normal(0, 1, (B, H, W, 3)).astype(np.float32)
     t = rng.normal(0, 0.5, (B, H, W, 2)).astype(np.float32)
@@ -51,7 +56,8 @@ def test_forward_backward_match_oracle():
     o_out, o_loss, o_g = U.forward_backward(p, c0, x, t, m, use_bf16=True)
     e_out = rel(out.numpy(), o_out)
     print("out rel", e_out, "loss", loss, o_loss)
-    assert e_out < 2e-2 and abs(loss - o_loss) <= 2e-2 * abs(o_loss)
+    # measured: output 3.2e-9, loss 4.7e-8 relative
+    assert e_out < 3e-8 and abs(loss - o_loss) <= 5e-7 * abs(o_loss)
     off = 0
     worst = 0.0
     for i, (cin, cout, k, relu) in enumerate(U.layer_table(c0)):
@@ -60,7 +66,7 @@ def test_forward_backward_match_oracle():
         worst = max(worst, e)
         print("layer", i, "grad rel", e)
         off += n
-    assert worst < 5e-2
+    assert worst < 1e-2  # measured: 9.5e-4 (first layer; BF16 operands round differently)
 
 
 def test_training_reduces_the_loss():
