@@ -55,9 +55,7 @@ struct mts_batch_s {
     double* acc = nullptr;
     double* part = nullptr;      // per block
     int32_t* d_first = nullptr;  // per lattice: first block id (nb + 1)
-    BatBlock* d_blocks = nullptr;
-    int32_t* d_queue = nullptr;
-    int32_t* d_nactive = nullptr;
+    BatBlock* d_queue = nullptr;  // the active blocks' records, padded to the grid
     int nactive = 0;
     // tile kernels (two steps per launch): tile-major cell order, per tile
     // row the fluid mask and first cell, per tile its 3 x 3 neighbourhood
@@ -125,15 +123,19 @@ struct mts_batch_s {
         return c;
     }
     int upload_queue() {
-        std::vector<int32_t> q, tq;
+        // the active blocks' records in queue order, padded with empty
+        // entries to the grid of a step
+        std::vector<BatBlock> q;
+        std::vector<int32_t> tq;
         for (int g = 0; g < nb; g++)
             if (active[g]) {
-                q.insert(q.end(), blocks_of[g].begin(), blocks_of[g].end());
+                for (int32_t bid : blocks_of[g]) q.push_back(hblocks[bid]);
                 if (tiled) tq.insert(tq.end(), tiles_of[g].begin(), tiles_of[g].end());
             }
         nactive = (int)q.size();
-        if (!q.empty()) BT_TRY(hipMemcpyAsync(d_queue, q.data(), q.size() * 4, hipMemcpyHostToDevice, stream));
-        BT_TRY(hipMemcpyAsync(d_nactive, &nactive, 4, hipMemcpyHostToDevice, stream));
+        q.resize((size_t)grid_for(nactive), BatBlock{0, 0, -1, -1});
+        if (!q.empty())
+            BT_TRY(hipMemcpyAsync(d_queue, q.data(), q.size() * sizeof(BatBlock), hipMemcpyHostToDevice, stream));
         if (tiled) {
             ntactive = (int)tq.size();
             if (!tq.empty()) BT_TRY(hipMemcpyAsync(d_tqueue, tq.data(), tq.size() * 4, hipMemcpyHostToDevice, stream));
@@ -143,7 +145,7 @@ struct mts_batch_s {
     }
     // Grid of a step: the active blocks rounded up to a power of two (at
     // least 1024), so that a shrinking queue is not launched over every
-    // block of the batch (blocks past the active count exit at once, but
+    // block of the batch (the queue's padding entries exit at once, but
     // hundreds of thousands of them still cost their dispatch), while the
     // captured chunk is replaced only when the bucket changes.
     int grid_for(int n) const {
@@ -211,9 +213,7 @@ struct mts_batch_s {
         L.c = ctx();
         L.Ain = A[cur];
         L.Aout = A[cur ^ 1];
-        L.blocks = d_blocks;
         L.queue = d_queue;
-        L.nactive = d_nactive;
         L.maxblocks = grid_for(nactive);
         L.mac = stats ? mac : nullptr;
         L.part = stats ? part : nullptr;
@@ -486,8 +486,10 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
             for (int j = 8; j < AUXW; j++) C.aux.push_back(r[j]);
         }
         for (int64_t s0 = 0; s0 < n; s0 += 256) {
-            b->blocks_of[g].push_back((int32_t)b->hblocks.size());
-            b->hblocks.push_back(BatBlock{(int32_t)(base + s0), (int32_t)(base + std::min<int64_t>(s0 + 256, n)), g, 0});
+            const int64_t s1 = std::min<int64_t>(s0 + 256, n);
+            const BatBlock B{(int32_t)(base + s0), (int32_t)(base + s1), g, (int32_t)b->hblocks.size()};
+            b->blocks_of[g].push_back(B.id);
+            b->hblocks.push_back(B);
         }
         base += ((n + 255) / 256) * 256;
         // padding cells between lattices are never in a block
@@ -527,8 +529,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
                                        (rc = b->alloc(&b->M[1], (size_t)base * 4)))) ||
         (rc = b->alloc(&b->part, 2 * b->hblocks.size())) ||
         (rc = b->alloc(&b->d_first, (size_t)nb + 1)) ||
-        (rc = b->alloc(&b->d_blocks, b->hblocks.size())) || (rc = b->alloc(&b->d_queue, b->hblocks.size())) ||
-        (rc = b->alloc(&b->d_nactive, 1)))
+        (rc = b->alloc(&b->d_queue, b->hblocks.size())))
         return bail(rc);
     e = hipMemcpyAsync(cw, C.cw.data(), base * 4, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(rw, C.rw.data(), base * 4, hipMemcpyHostToDevice, b->stream);
@@ -539,9 +540,6 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     if (e == hipSuccess)
         e = hipMemcpyAsync(cpar, b->cpar_h.data(), b->cpar_h.size() * 4, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(b->book, book.data(), book.size() * 4, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->d_blocks, b->hblocks.data(), b->hblocks.size() * sizeof(BatBlock), hipMemcpyHostToDevice,
-                           b->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b->mac, 0, (size_t)base * 3 * 4, b->stream);
     std::vector<int32_t> first(nb + 1);
     for (int g = 0; g < nb; g++) first[g] = b->blocks_of[g].empty() ? 0 : b->blocks_of[g][0];

@@ -155,17 +155,18 @@ int launch_cmultistep(const LaunchCtx& c, int mode, int mac, const MbSpec& s);
 int launch_cinterface(const LaunchCtx& c1, const LaunchCtx& c2, const IfSpec& s);
 // Batched single-block lattices (cell order): one block of 256 cells of one
 // lattice per queue entry.
+// A 256-cell block of the batched image: cells [start, end) of lattice
+// geom; id = its index among the batch's blocks.  An entry with start == end
+// pads the work queue to the grid size.
 struct BatBlock {
-    int32_t start, end, geom, pad;
+    int32_t start, end, geom, id;
 };
 struct BatchLaunch {
     LaunchCtx c;  // cimg: the concatenated image
     const float* Ain = nullptr;
     float* Aout = nullptr;
-    const BatBlock* blocks = nullptr;
-    const int32_t* queue = nullptr;
-    const int32_t* nactive = nullptr;  // device scalar
-    int maxblocks = 0;                 // grid size (blocks beyond *nactive exit)
+    const BatBlock* queue = nullptr;  // maxblocks entries: the active blocks, then padding
+    int maxblocks = 0;                // grid size
     float* mac = nullptr;
     double* part = nullptr;  // per-block (sum rho, sum j_x)
     int from_rest = 0;
