@@ -129,6 +129,17 @@ int mts_grid_arrays(mts_grid g, int64_t* gridID, int32_t* gridIJ, int32_t* level
  * call with rows = NULL to get *count (rows). */
 int mts_grid_rows(mts_grid g, int which, int64_t* rows, int64_t* count);
 
+/* ---- Quadtree tables on the device (no counterpart in the reference,
+ * which builds them on the host in grid.hpp:313-1157).  mts_create of a
+ * multi-block lattice makes the level decision and the neighbour and
+ * interface tables (connectivity, boundary types, parent/child, coarse and
+ * fine buffer rows) and the level images' link words on the device; the
+ * buffer cells, whose placement is sequential, stay on the host.
+ * mts_quadtree_check builds one geometry both ways (the host builder behind
+ * mts_grid_create, and the device builder) and sets *ndiff to the number of
+ * table entries that differ (0: identical).  Needs a GPU. */
+int mts_quadtree_check(int nx, int ny, const int32_t* pore, const double* kn, const mts_params* p, int64_t* ndiff);
+
 /* Grid2D::globalGridSize (grid.hpp:195). */
 int mts_cell_count(mts_sim s, int64_t* n);
 

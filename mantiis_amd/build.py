@@ -38,7 +38,7 @@ def build_lib(verbose=True):
         o = os.path.join(BUILD, src.replace(".cpp", ".o"))
         cmds.append(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    for src in ("mts_kernels.hip", "mts_api.hip", "mts_batch.hip", "mts_net.hip", "mts_shape.hip"):
+    for src in ("mts_kernels.hip", "mts_api.hip", "mts_batch.hip", "mts_net.hip", "mts_shape.hip", "mts_quadtree.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

@@ -697,14 +697,20 @@ static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const
     s->multi = true;
     hipError_t e = hipSetDevice(s->prm.device);
     if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
-    int rc = build_multi_grid(nx, ny, pore, kn, locpore, s->ph, s->prm.periodic_y != 0, s->mg, err);
-    if (rc) return rc;
-    rc = multi_grid_taus(s->ph, s->mg, err);
-    if (rc) return rc;
-    rc = build_level_images(s->mg, s->limg, s->mparams, s->ph, err);
-    if (rc) return rc;
     e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
+    // the quadtree: level decision and tables on the device, the buffer
+    // cells (a sequential rule) and the parameter table on the host
+    (void)locpore;
+    const bool torus = s->prm.periodic_y != 0;
+    std::vector<uint8_t> coarse;
+    int rc = multi_check(nx, ny, pore, err);
+    if (rc) return rc;
+    if ((rc = multi_coarse_device(nx, ny, pore, torus, s->stream, coarse, err))) return rc;
+    if ((rc = multi_cells(nx, ny, pore, kn, torus, coarse, s->mg, err))) return rc;
+    if ((rc = multi_grid_taus(s->ph, s->mg, err))) return rc;
+    if ((rc = build_level_images(s->mg, s->limg, s->mparams, s->ph, err, false))) return rc;
+    if ((rc = multi_tables_device(s->mg, coarse, &s->limg, s->stream, err))) return rc;
     const size_t esz = s->prm.precision == MTS_FP32DEV ? 4 : 8;
     int64_t Pmax = 0;
     s->geo.nx = nx;

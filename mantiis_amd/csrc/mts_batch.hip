@@ -7,10 +7,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mantiis_hip.h"
@@ -407,15 +409,44 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
     std::vector<uint32_t> htmask;
     CompactImage& C = b->cat;
     int64_t base = 0;
-    for (int g = 0; g < nb; g++) {
-        if (nx[g] <= 0 || ny[g] <= 0 || !pore[g] || !kn[g] || !locpore[g])
-            return bail(set_error(MTS_ERR_ARG, "lattice " + std::to_string(g) + ": bad geometry"));
+    // The lattices' grids and cell images are independent: host threads
+    // build them a chunk at a time, then they are appended in lattice order.
+    struct Built {
         Geometry geo;
-        std::string err;
-        int rc = build_single_grid(nx[g], ny[g], pore[g], kn[g], locpore[g], b->ph, true, geo, err);
-        if (rc) return bail(set_error(rc, "lattice " + std::to_string(g) + ": " + err));
         CompactImage ci;
-        build_compact(geo, ci);
+        int rc = MTS_OK;
+        std::string err;
+    };
+    const int nthr = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    const int chunk = 4 * nthr;
+    std::vector<Built> built;
+    for (int g = 0; g < nb; g++) {
+        if (g % chunk == 0) {
+            const int g0 = g, g1 = std::min(nb, g + chunk);
+            built.clear();
+            built.resize(g1 - g0);
+            std::atomic<int> next{g0};
+            auto work = [&]() {
+                for (int h; (h = next.fetch_add(1)) < g1;) {
+                    Built& B = built[h - g0];
+                    if (nx[h] <= 0 || ny[h] <= 0 || !pore[h] || !kn[h] || !locpore[h]) {
+                        B.rc = MTS_ERR_ARG;
+                        B.err = "bad geometry";
+                        continue;
+                    }
+                    B.rc = build_single_grid(nx[h], ny[h], pore[h], kn[h], locpore[h], b->ph, true, B.geo, B.err);
+                    if (B.rc == MTS_OK) build_compact(B.geo, B.ci);
+                }
+            };
+            std::vector<std::thread> pool;
+            for (int i = 1; i < std::min(nthr, g1 - g0); i++) pool.emplace_back(work);
+            work();
+            for (auto& t : pool) t.join();
+        }
+        Built& B = built[g % chunk];
+        if (B.rc) return bail(set_error(B.rc, "lattice " + std::to_string(g) + ": " + B.err));
+        const Geometry& geo = B.geo;
+        const CompactImage& ci = B.ci;
         const int32_t poff = (int32_t)b->params.size();
         if (b->params.size() + geo.params.size() > 65536)
             return bail(set_error(MTS_ERR_UNSUPPORTED, "more than 65536 distinct (tau_s, Kn) pairs in the batch"));
@@ -498,6 +529,7 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
         C.pidx.resize(base, 0);
         C.outm.resize(base, 0);
         if (g == 0) canonical_spec(C.spec);
+        B = Built();  // release the lattice's host tables
     }
     b->tfirst[nb] = b->ntiles;
     C.ncell = base;

@@ -95,9 +95,7 @@ int wrapc(int v, int n) { return ((v % n) + n) % n; }
 
 }  // namespace
 
-int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore, const Physics& ph,
-                     bool torus, MultiGeometry& G, std::string& err) {
-    G = MultiGeometry();
+int multi_check(int nx, int ny, const int32_t* pore, std::string& err) {
     if (nx != ny || nx < 8 || (nx & (nx - 1)) != 0) {
         err = "Matrix dimensions must be divisible by 2 in each direction. (multi-block lattices are square, "
               "power-of-two sided)";
@@ -108,18 +106,19 @@ int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, cons
             err = "pore image must hold only 0 (solid) and 1 (fluid)";
             return MTS_ERR_ARG;
         }
-    G.nx = nx;
-    G.ny = ny;
-    G.torus = torus;
-    // ---- level decision: aligned 2x2 fluid block, wall distance >= 6
+    return MTS_OK;
+}
+
+// Level decision: an aligned 2x2 block is coarse when its four pixels are
+// fluid at a Euclidean distance >= 6 from every wall pixel.
+void multi_coarse_host(int nx, int ny, const int32_t* pore, bool torus, std::vector<uint8_t>& coarse) {
     const std::vector<double> d2 = edt2(nx, ny, pore, torus);
     auto dist = [&](int x, int y) -> double {
         const double v = d2[(size_t)y * nx + x];
         return v >= 1e299 ? 1e30 : (double)(float)std::sqrt(v);
     };
     const int cnx = nx / 2, cny = ny / 2;
-    std::vector<uint8_t> coarse((size_t)cnx * cny, 0);
-    bool any = false;
+    coarse.assign((size_t)cnx * cny, 0);
     for (int by = 0; by < cny; by++)
         for (int bx = 0; bx < cnx; bx++) {
             bool ok = true;
@@ -129,8 +128,35 @@ int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, cons
                     ok = pore[(size_t)y * nx + x] == 1 && dist(x, y) >= 6.0;
                 }
             coarse[(size_t)by * cnx + bx] = ok;
-            any = any || ok;
         }
+}
+
+int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore, const Physics& ph,
+                     bool torus, MultiGeometry& G, std::string& err) {
+    (void)ph;
+    (void)locpore;
+    G = MultiGeometry();
+    if (int rc = multi_check(nx, ny, pore, err)) return rc;
+    std::vector<uint8_t> coarse;
+    multi_coarse_host(nx, ny, pore, torus, coarse);
+    if (int rc = multi_cells(nx, ny, pore, kn, torus, coarse, G, err)) return rc;
+    return multi_tables_host(G, coarse, err);
+}
+
+// The cells of the quadtree: original cells (fluid pixels outside coarse
+// blocks, coarse blocks) in row-major order of their top-left pixel, then
+// the buffer cells in creation order (grid.hpp:427-889; sequential: a
+// buffer's placement depends on the buffers made before it).
+int multi_cells(int nx, int ny, const int32_t* pore, const double* kn, bool torus, const std::vector<uint8_t>& coarse,
+                MultiGeometry& G, std::string& err) {
+    (void)err;
+    G = MultiGeometry();
+    G.nx = nx;
+    G.ny = ny;
+    G.torus = torus;
+    const int cnx = nx / 2, cny = ny / 2;
+    bool any = false;
+    for (uint8_t v : coarse) any = any || v;
     G.maxLevel = any ? 2 : 1;
     const int L = G.maxLevel;
     auto gid = [&](int x, int y, int lvl) -> int64_t { return (int64_t)x + (int64_t)y * ny + (int64_t)(lvl - 1) * ny * nx; };
@@ -276,6 +302,21 @@ int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, cons
     G.ncell = n;
     G.gridID.resize(n);
     for (int64_t c = 0; c < n; c++) G.gridID[c] = gid(G.cx[c], G.cy[c], G.level[c]);
+    // per-cell Kn (the loaded map at the cell's first pixel); the
+    // relaxation times need the device (multi_grid_taus)
+    G.kn.resize(n);
+    for (int64_t c = 0; c < n; c++) G.kn[c] = kn[(size_t)G.cy[c] * nx + G.cx[c]];
+    G.taus.clear();
+    G.pore.assign(pore, pore + (size_t)nx * ny);
+    return MTS_OK;
+}
+
+// Neighbour and interface tables of the cells (grid.hpp:916-1157):
+// connectivity, boundary types, parent/child rows and the coarse and fine
+// buffer rows.  The device builder (mts_quadtree.hip) forms the same tables.
+int multi_tables_host(MultiGeometry& G, const std::vector<uint8_t>& coarse, std::string& err) {
+    const int nx = G.nx, ny = G.ny, cnx = nx / 2, cny = ny / 2, L = G.maxLevel;
+    const int64_t n = G.ncell;
     // per-level index images: cell at a level position
     std::vector<int64_t> at1((size_t)nx * ny, -1), at2((size_t)cnx * cny, -1);
     std::vector<uint8_t> orig1((size_t)nx * ny, 0), orig2((size_t)cnx * cny, 0);
@@ -283,10 +324,14 @@ int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, cons
         if (G.level[c] == 1) at1[(size_t)G.cy[c] * nx + G.cx[c]] = c;
         else at2[(size_t)(G.cy[c] / 2) * cnx + G.cx[c] / 2] = c;
     }
-    for (const Orig& o : orig) {  // positions of linInd (incl. wall pixels) per level
-        if (o.lvl == 1) orig1[(size_t)o.y * nx + o.x] = 1;
-        else orig2[(size_t)(o.y / 2) * cnx + o.x / 2] = 1;
-    }
+    // positions of original cells (incl. wall pixels) per level: a pixel
+    // outside the coarse blocks, a coarse block
+    for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+            const bool inCoarse = L > 1 && coarse[(size_t)(y / 2) * cnx + x / 2];
+            if (inCoarse) orig2[(size_t)(y / 2) * cnx + x / 2] = 1;
+            else orig1[(size_t)y * nx + x] = 1;
+        }
     // ---- connectivity (grid.hpp:916-972)
     G.connect.assign(n * 9, -2);
     for (int64_t c = 0; c < n; c++) {
@@ -358,13 +403,6 @@ int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, cons
             G.fineRows.insert(G.fineRows.end(), row, row + 5);
         }
     }
-    // ---- per-cell Kn (the loaded map at the cell's first pixel); the
-    // relaxation times need the device (multi_grid_taus)
-    G.kn.resize(n);
-    for (int64_t c = 0; c < n; c++) G.kn[c] = kn[(size_t)G.cy[c] * nx + G.cx[c]];
-    G.taus.clear();
-    G.pore.assign(pore, pore + (size_t)nx * ny);
-    (void)locpore;
     return MTS_OK;
 }
 
@@ -424,7 +462,7 @@ int multi_grid_taus(const Physics& ph, MultiGeometry& G, std::string& err) {
 namespace mts {
 
 int build_level_images(const MultiGeometry& G, std::vector<LevelImage>& Ls, std::vector<CellParam>& params,
-                       const Physics& ph, std::string& err) {
+                       const Physics& ph, std::string& err, bool meta) {
     Ls.assign(G.maxLevel, LevelImage());
     for (int l = 0; l < G.maxLevel; l++) {
         LevelImage& L = Ls[l];
@@ -463,6 +501,7 @@ int build_level_images(const MultiGeometry& G, std::vector<LevelImage>& Ls, std:
             id = it->second;
         L.pidx[s] = (uint16_t)id;
     }
+    if (!meta) return MTS_OK;  // the link words come from the device builder
     const int half = 1 << (G.maxLevel - 1);
     for (int64_t c = 0; c < G.ncell; c++) {
         LevelImage& L = Ls[G.level[c] - 1];

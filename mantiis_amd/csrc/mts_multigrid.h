@@ -26,6 +26,15 @@ struct MultiGeometry {
 
 int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore, const Physics& ph,
                      bool torus, MultiGeometry& G, std::string& err);
+// The stages of build_multi_grid: argument check, level decision (coarse
+// flags of the aligned 2x2 blocks, cnx * cny), the cells (original and
+// buffer cells, their positions, levels, types, gridID, Kn, the pore image)
+// and their neighbour and interface tables.
+int multi_check(int nx, int ny, const int32_t* pore, std::string& err);
+void multi_coarse_host(int nx, int ny, const int32_t* pore, bool torus, std::vector<uint8_t>& coarse);
+int multi_cells(int nx, int ny, const int32_t* pore, const double* kn, bool torus, const std::vector<uint8_t>& coarse,
+                MultiGeometry& G, std::string& err);
+int multi_tables_host(MultiGeometry& G, const std::vector<uint8_t>& coarse, std::string& err);
 // Per-cell relaxation times of a built grid (needs the current device).
 int multi_grid_taus(const Physics& ph, MultiGeometry& G, std::string& err);
 
@@ -46,7 +55,24 @@ struct LevelImage {
 
 // Link words and parameter indices of every level image (the classification
 // of coreLB.hpp:292-356 applied to the multi-block connectivity).
+// meta = false leaves the link words at zero (the device builder forms them).
 int build_level_images(const MultiGeometry& G, std::vector<LevelImage>& L, std::vector<CellParam>& params,
-                       const Physics& ph, std::string& err);
+                       const Physics& ph, std::string& err, bool meta = true);
+
+}  // namespace mts
+
+// the HIP stream type (this header is also compiled by the host compiler)
+typedef struct ihipStream_t* hipStream_t;
+
+namespace mts {
+
+// The device builder (mts_quadtree.hip): the level decision, and, from the
+// host's cells, the neighbour and interface tables of G and (levels !=
+// nullptr, images sized by build_level_images) the link words of the level
+// images -- entry for entry those of the host builder.
+int multi_coarse_device(int nx, int ny, const int32_t* pore, bool torus, hipStream_t st, std::vector<uint8_t>& coarse,
+                        std::string& err);
+int multi_tables_device(MultiGeometry& G, const std::vector<uint8_t>& coarse, std::vector<LevelImage>* levels,
+                        hipStream_t st, std::string& err);
 
 }  // namespace mts

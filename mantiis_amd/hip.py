@@ -31,7 +31,7 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_set_body_force", "mts_initialize", "mts_step_transport", "mts_step_adsorption",
            "mts_step_coupled", "mts_residual", "mts_method", "mts_field_size", "mts_get_field",
            "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_permeability", "mts_get_stream", "mts_synchronize",
-           "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows",
+           "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows", "mts_quadtree_check",
            "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy", "mts_probe_copy16", "mts_batch_create", "mts_batch_destroy",
            "mts_batch_cells", "mts_batch_run", "mts_batch_step", "mts_batch_kernel", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
            "mts_batch_stream", "mts_batch_tune", "mts_batch_set_config", "mts_dev_alloc", "mts_dev_free", "mts_dev_upload", "mts_dev_download",
@@ -585,6 +585,22 @@ def gemm_check(A, B):
     Cm = np.zeros((M, N), np.float32)
     _check(lib, lib.mts_net_gemm_check(A.ctypes.data_as(f), B.ctypes.data_as(f), M, N, K, Cm.ctypes.data_as(f)))
     return Cm
+
+
+def quadtree_check(pore, kn, **kw):
+    """Builds the quadtree of `pore` with the host builder and with the
+    device builder (mts_quadtree_check) and returns the number of table
+    entries that differ (level decision, cells, connectivity, boundary
+    types, parent/child and buffer rows, level-image link words)."""
+    lib = load_library()
+    lib.mts_quadtree_check.argtypes = [C.c_int, C.c_int, _I32, _D, C.POINTER(Params), _I64]
+    pore = np.ascontiguousarray(pore, np.int32)
+    kn = np.ascontiguousarray(kn, np.float64)
+    prm = default_params(**kw)
+    n = C.c_int64(-1)
+    _check(lib, lib.mts_quadtree_check(pore.shape[1], pore.shape[0], _p(pore, _I32), _p(kn, _D), C.byref(prm),
+                                       C.byref(n)))
+    return n.value
 
 
 def shape_properties(domain, resolution, mfp, skeleton=False):

@@ -157,3 +157,36 @@ def test_multiblock_taus_per_level():
                 want = 0.5 + np.sqrt(6 / np.pi) * lpL * knL / (1 + 2 * knL)
                 assert (lvl == L).any() and np.allclose(t[lvl == L], want, rtol=1e-14), L
         g.close()
+
+
+TABLE_CASES = [
+    ("box64_reference", box64, False),
+    ("disks128_torus", lambda: G.disk_packing(128, 0.7, 3, 8, seed=3), True),
+    ("disks256_reference", lambda: G.disk_packing(256, 0.6, 4, 12, seed=8), False),
+    ("disks512_torus", lambda: G.disk_packing(512, 0.55, 3, 16, seed=9), True),
+    ("all_fluid64_torus", lambda: np.ones((64, 64), np.int32), True),
+    ("narrow_pores64", lambda: G.disk_packing(64, 0.4, 2, 3, seed=2), True),
+]
+
+
+@pytest.mark.parametrize("name,geom,torus", TABLE_CASES, ids=[c[0] for c in TABLE_CASES])
+def test_device_quadtree_tables_equal_host(name, geom, torus):
+    """The quadtree tables built on the device (level decision by a window
+    test, connectivity, boundary types, parent/child and buffer rows, level
+    image link words) equal the host builder's entry for entry; the
+    lattices of this file's other tests are built by the device builder and
+    match the oracle."""
+    from mantiis_amd.hip import quadtree_check
+    banner(f"quadtree tables device vs host: {name}")
+    pore = geom()
+    kn, _ = G.uniform_kn(pore, 0.1, MFP_DX)
+    assert quadtree_check(pore, kn, periodic_y=int(torus)) == 0
+
+
+def test_device_quadtree_tables_at_config_size():
+    """The 1024^2 lattice of the quadtree benchmark configuration."""
+    import bench
+    from mantiis_amd.hip import quadtree_check
+    banner("quadtree tables device vs host: 1024^2 benchmark lattice")
+    pore, kn, lp, phi = bench.quadtree_inputs(1024)
+    assert quadtree_check(pore, kn, periodic_y=1) == 0
