@@ -451,13 +451,16 @@ def bench_warmstart(args):
     rho_in = globals_for(dx=1e-9)[1]["rhoin"]
     t0 = time.perf_counter()
     train_s = WS.samples(range(10000, 10000 + args.train), 256)
+    print(f"[bench] {len(train_s)} training lattices generated", file=sys.stderr, flush=True)
     r, fields = WS.solve(train_s, F)
     t_data = time.perf_counter() - t0
+    print(f"[bench] training lattices solved to steady state ({t_data:.1f} s)", file=sys.stderr, flush=True)
     ds = WS.Dataset([WS.features(s, F) for s in train_s], [WS.targets(s, f, F) for s, f in zip(train_s, fields)])
     del fields
     c0, B = 24, 8
     t0 = time.perf_counter()
-    net, hist = WS.train(ds, c0=c0, batch=B, epochs=args.epochs, lr=1e-3)
+    net, hist = WS.train(ds, c0=c0, batch=B, epochs=args.epochs, lr=1e-3,
+                         log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
     t_train = time.perf_counter() - t0
     steps_train = args.epochs * (ds.n // B)
     flops = net.flops(B, 256, 256)
@@ -467,16 +470,17 @@ def bench_warmstart(args):
     ds = None
     eval_s = WS.samples(range(20000, 20000 + args.eval), 512)
     t0 = time.perf_counter()
+    print(f"[bench] {len(eval_s)} evaluation lattices generated", file=sys.stderr, flush=True)
     rc, fc = WS.solve(eval_s, F)
+    print("[bench] evaluation lattices solved from rest", file=sys.stderr, flush=True)
     t_cold = time.perf_counter() - t0
     t0 = time.perf_counter()
     pred = WS.predict(params, c0, eval_s, F)
     t_pred = time.perf_counter() - t0
     t0 = time.perf_counter()
-    warm = [WS.seed_populations(s, ux, uy, rho_in) for s, (ux, uy) in zip(eval_s, pred)]
-    t_seed = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    rw, fw = WS.solve(eval_s, F, warm=warm)
+    t_seed = 0.0  # the populations are formed on the device inside the warm-started solve (mts_batch_seed)
+    print("[bench] predicted", file=sys.stderr, flush=True)
+    rw, fw = WS.solve(eval_s, F, warm_u=pred, rho_in=rho_in)
     t_warm = time.perf_counter() - t0
     kc = np.array([WS.permeability(s, f[1], F) for s, f in zip(eval_s, fc)])
     kw = np.array([WS.permeability(s, f[1], F) for s, f in zip(eval_s, fw)])

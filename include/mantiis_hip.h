@@ -295,7 +295,7 @@ int mts_batch_run(mts_batch b, int64_t max_steps, int check_every, double tol, i
  * (creation or the last check of mts_batch_run). */
 int mts_batch_step(mts_batch b, int nsteps);
 /* the batch's step kernel: *steps_per_launch = 2 for the tile kernels, 1 for
- * the one-step cell-order kernel; *units = its tiles (32 x 32 sites) or
+ * the one-step cell-order kernel; *units = its tiles (32 x 16 sites) or
  * 256-cell blocks (either pointer may be NULL) */
 int mts_batch_kernel(mts_batch b, int* steps_per_launch, int64_t* units);
 /* marks lattices active (1) or not (0) for the next runs */
@@ -306,6 +306,17 @@ int mts_batch_fields(mts_batch b, int g, double* rho, double* ux, double* uy);
 /* sets lattice g's post-stream populations (9 planes of nx*ny doubles,
  * f[j*nx*ny + site]; solids ignored): a warm start */
 int mts_batch_set_state(mts_batch b, int g, const double* f);
+/* the same for n lattices g[0..n-1] with populations f[0..n-1], collided in
+ * one launch */
+int mts_batch_set_states(mts_batch b, int n, const int32_t* g, const double* const* f);
+/* warm start from velocity fields (no counterpart in the reference):
+ * lattices g[0..n-1] get, on the device, the populations of the equilibrium
+ * at rho plus the regularised non-equilibrium part -3 tau[i] w_j rho
+ * (Q_j : grad u) of the given u (ux[i], uy[i]: nx*ny row-major doubles,
+ * zero in the solids; central periodic differences), collided in one
+ * launch like mts_batch_set_states */
+int mts_batch_seed(mts_batch b, int n, const int32_t* g, const double* const* ux, const double* const* uy,
+                   const double* tau, double rho);
 int mts_batch_stream(mts_batch b, void** stream);
 /* Autotuner (no reference counterpart): steps_each > 0 times every launch
  * configuration of the batched step -- threads per 256-cell block {64, 128,

@@ -171,7 +171,10 @@ struct mts_sim_s {
     int alloc(T** p, size_t n) {
         void* q = nullptr;
         hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
-        if (e != hipSuccess) return fail(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // a failed allocation must not surface at a later launch check
+            return fail(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
         e = hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(T), stream);
         if (e != hipSuccess) return fail(MTS_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
         allocs.push_back(q);
@@ -1736,7 +1739,11 @@ int mts_dev_alloc(size_t bytes, void** ptr) {
     if (!ptr || bytes == 0) return fail(MTS_ERR_ARG, "bad argument");
     *ptr = nullptr;
     hipError_t e = hipMalloc(ptr, bytes);
-    if (e != hipSuccess) return fail(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // a failed allocation must not surface at a later launch check
+        *ptr = nullptr;
+        return fail(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
     return MTS_OK;
 }
 

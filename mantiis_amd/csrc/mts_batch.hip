@@ -94,7 +94,10 @@ struct mts_batch_s {
     int alloc(T** p, size_t n) {
         void* q = nullptr;
         hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
-        if (e != hipSuccess) return set_error(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // a failed allocation must not surface at a later launch check
+            return set_error(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
         allocs.push_back(q);
         *p = (T*)q;
         return MTS_OK;
@@ -768,31 +771,147 @@ int mts_batch_fields(mts_batch b, int g, double* rho, double* ux, double* uy) {
     return MTS_OK;
 }
 
-int mts_batch_set_state(mts_batch b, int g, const double* f) {
+int mts_batch_set_states(mts_batch b, int n, const int32_t* gs, const double* const* f) {
     if (int rc = check(b)) return rc;
-    if (g < 0 || g >= b->nb || !f) return set_error(MTS_ERR_ARG, "bad argument");
-    const int64_t n = b->ncell[g], o = b->off[g], S = b->dev.cstride;
-    const int64_t P = (int64_t)b->nx[g] * b->ny[g];
+    if (n <= 0 || !gs || !f) return set_error(MTS_ERR_ARG, "bad argument");
+    for (int i = 0; i < n; i++)
+        if (gs[i] < 0 || gs[i] >= b->nb || !f[i]) return set_error(MTS_ERR_ARG, "bad argument");
     static const double w[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
-    std::vector<float> h(9 * n);
-    const auto& cp = b->cellPos[g];
-    for (int64_t c = 0; c < n; c++)
-        for (int j = 0; j < 9; j++) h[j * n + c] = (float)(f[(int64_t)j * P + cp[c]] - w[j] * b->ph.rhoin);
-    // the given post-stream state goes to the other buffer; the lattice's
-    // blocks alone then collide it into the carried state
-    for (int j = 0; j < 9; j++)
-        BT_TRY(hipMemcpyAsync(b->A[b->cur ^ 1] + j * S + o, &h[j * n], n * 4, hipMemcpyHostToDevice, b->stream));
+    const int64_t S = b->dev.cstride;
+    // the given post-stream states go to the other buffer (deviations from
+    // w rho0 in cell order, converted on host threads); the lattices' blocks
+    // alone then collide them into the carried state in one launch
+    std::vector<std::vector<float>> h(n);
+    {
+        std::atomic<int> next{0};
+        auto work = [&]() {
+            for (int i; (i = next.fetch_add(1)) < n;) {
+                const int g = gs[i];
+                const int64_t nc = b->ncell[g], P = (int64_t)b->nx[g] * b->ny[g];
+                const auto& cp = b->cellPos[g];
+                h[i].resize(9 * nc);
+                for (int64_t c = 0; c < nc; c++)
+                    for (int j = 0; j < 9; j++) h[i][j * nc + c] = (float)(f[i][(int64_t)j * P + cp[c]] - w[j] * b->ph.rhoin);
+            }
+        };
+        const int nthr = std::max(1, std::min({16, n, (int)std::thread::hardware_concurrency()}));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nthr; t++) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+    }
+    for (int i = 0; i < n; i++) {
+        const int g = gs[i];
+        const int64_t nc = b->ncell[g], o = b->off[g];
+        for (int j = 0; j < 9; j++)
+            BT_TRY(hipMemcpyAsync(b->A[b->cur ^ 1] + j * S + o, &h[i][j * nc], nc * 4, hipMemcpyHostToDevice, b->stream));
+    }
     std::vector<char> keep = b->active;
     std::fill(b->active.begin(), b->active.end(), 0);
-    b->active[g] = 1;
+    for (int i = 0; i < n; i++) b->active[gs[i]] = 1;
     int rc;
     if ((rc = b->upload_queue())) return rc;
     if ((rc = b->step(true, 2))) return rc;
     b->active = keep;
-    b->u_prev[g] = 0.0;
+    for (int i = 0; i < n; i++) b->u_prev[gs[i]] = 0.0;
+    if ((rc = b->upload_queue())) return rc;
+    BT_TRY(hipStreamSynchronize(b->stream));  // the host copies go out of scope
+    return MTS_OK;
+}
+
+namespace {
+
+// Warm-start populations of one lattice from a velocity field (row-major
+// site images, zero in the solids): equilibrium at rho plus the regularised
+// non-equilibrium part -3 tau w_i rho (Q_i : grad u) with central periodic
+// differences, written as deviations from w_i rho0 into the lattice's cells
+// of the 9 planes at A (plane stride S).
+__global__ void k_seed(int nx, int ny, int64_t ncell, const int32_t* __restrict__ cellPos, const double* __restrict__ ux,
+                       const double* __restrict__ uy, double tau, double rho, double rho0, float* __restrict__ A,
+                       int64_t S) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int p = cellPos[c], x = p % nx, y = p / nx;
+    const int xp = x + 1 == nx ? 0 : x + 1, xm = x == 0 ? nx - 1 : x - 1;
+    const int yp = y + 1 == ny ? 0 : y + 1, ym = y == 0 ? ny - 1 : y - 1;
+    const double u = ux[p], v = uy[p];
+    const double dxux = (ux[(int64_t)y * nx + xp] - ux[(int64_t)y * nx + xm]) / 2;
+    const double dyuy = (uy[(int64_t)yp * nx + x] - uy[(int64_t)ym * nx + x]) / 2;
+    const double dxuy = (uy[(int64_t)y * nx + xp] - uy[(int64_t)y * nx + xm]) / 2;
+    const double dyux = (ux[(int64_t)yp * nx + x] - ux[(int64_t)ym * nx + x]) / 2;
+    const double usq = u * u + v * v;
+    const int ex[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1}, ey[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    const double w[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+    for (int j = 0; j < 9; j++) {
+        const double cu = ex[j] * u + ey[j] * v;
+        const double feq = w[j] * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * usq);
+        const double q = (ex[j] * ex[j] - 1.0 / 3) * dxux + (ey[j] * ey[j] - 1.0 / 3) * dyuy + ex[j] * ey[j] * (dxuy + dyux);
+        const double f = feq - 3 * tau * w[j] * rho * q;
+        A[j * S + c] = (float)(f - w[j] * rho0);
+    }
+}
+
+}  // namespace
+
+int mts_batch_seed(mts_batch b, int n, const int32_t* gs, const double* const* ux, const double* const* uy,
+                   const double* tau, double rho) {
+    if (int rc = check(b)) return rc;
+    if (n <= 0 || !gs || !ux || !uy || !tau) return set_error(MTS_ERR_ARG, "bad argument");
+    for (int i = 0; i < n; i++)
+        if (gs[i] < 0 || gs[i] >= b->nb || !ux[i] || !uy[i]) return set_error(MTS_ERR_ARG, "bad argument");
+    // one staging buffer for the largest lattice: velocities and cell sites
+    int64_t Pmax = 1, nmax = 1;
+    for (int i = 0; i < n; i++) {
+        Pmax = std::max<int64_t>(Pmax, (int64_t)b->nx[gs[i]] * b->ny[gs[i]]);
+        nmax = std::max<int64_t>(nmax, b->ncell[gs[i]]);
+    }
+    double *d_u = nullptr, *d_v = nullptr;
+    int32_t* d_pos = nullptr;
+    hipError_t e = hipMalloc(&d_u, Pmax * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_v, Pmax * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_pos, nmax * 4);
+    float* dst = b->A[b->cur ^ 1];
+    for (int i = 0; i < n && e == hipSuccess; i++) {
+        const int g = gs[i];
+        const int64_t P = (int64_t)b->nx[g] * b->ny[g], nc = b->ncell[g];
+        e = hipMemcpyAsync(d_u, ux[i], P * 8, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_v, uy[i], P * 8, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_pos, b->cellPos[g].data(), nc * 4, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess) {
+            k_seed<<<(unsigned)((nc + 255) / 256), 256, 0, b->stream>>>(b->nx[g], b->ny[g], nc, d_pos, d_u, d_v, tau[i],
+                                                                          rho, b->ph.rhoin, dst + b->off[g],
+                                                                          b->dev.cstride);
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (d_u) (void)hipFree(d_u);
+    if (d_v) (void)hipFree(d_v);
+    if (d_pos) (void)hipFree(d_pos);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(e == hipErrorOutOfMemory ? MTS_ERR_OOM : MTS_ERR_HIP,
+                         std::string("warm-start seeding: ") + hipGetErrorString(e));
+    }
+    // the seeded post-stream states are collided in one launch
+    std::vector<char> keep = b->active;
+    std::fill(b->active.begin(), b->active.end(), 0);
+    for (int i = 0; i < n; i++) b->active[gs[i]] = 1;
+    int rc;
+    if ((rc = b->upload_queue())) return rc;
+    if ((rc = b->step(true, 2))) return rc;
+    b->active = keep;
+    for (int i = 0; i < n; i++) b->u_prev[gs[i]] = 0.0;
     if ((rc = b->upload_queue())) return rc;
     BT_TRY(hipStreamSynchronize(b->stream));
     return MTS_OK;
+}
+
+int mts_batch_set_state(mts_batch b, int g, const double* f) {
+    const int32_t gs[1] = {g};
+    const double* fs[1] = {f};
+    return mts_batch_set_states(b, 1, gs, fs);
 }
 
 int mts_batch_stream(mts_batch b, void** stream) {

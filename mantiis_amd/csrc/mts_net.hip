@@ -487,7 +487,10 @@ struct mts_net_s {
     int alloc(Tp** p, size_t n) {
         void* q = nullptr;
         hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(Tp));
-        if (e != hipSuccess) return set_error(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // a failed allocation must not surface at a later launch check
+            return set_error(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
         allocs.push_back(q);
         *p = (Tp*)q;
         return MTS_OK;

@@ -559,7 +559,10 @@ struct mts_shape_s {
     int alloc(T** p, size_t n) {
         void* q = nullptr;
         hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
-        if (e != hipSuccess) return set_error(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // a failed allocation must not surface at a later launch check
+            return set_error(MTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
         allocs.push_back(q);
         *p = (T*)q;
         return MTS_OK;

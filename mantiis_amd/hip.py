@@ -33,7 +33,7 @@ SYMBOLS = ["mts_default_params", "mts_last_error", "mts_globals", "mts_load_exis
            "mts_set_field", "mts_complete_domain", "mts_diagnostics", "mts_permeability", "mts_get_stream", "mts_synchronize",
            "mts_grid_create", "mts_grid_destroy", "mts_grid_size", "mts_grid_arrays", "mts_grid_rows", "mts_quadtree_check",
            "mts_kernel_timer", "mts_kernel_timer_read", "mts_probe_copy", "mts_probe_copy16", "mts_batch_create", "mts_batch_destroy",
-           "mts_batch_cells", "mts_batch_run", "mts_batch_step", "mts_batch_kernel", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state",
+           "mts_batch_cells", "mts_batch_run", "mts_batch_step", "mts_batch_kernel", "mts_batch_activate", "mts_batch_fields", "mts_batch_set_state", "mts_batch_set_states", "mts_batch_seed",
            "mts_batch_stream", "mts_batch_tune", "mts_batch_set_config", "mts_dev_alloc", "mts_dev_free", "mts_dev_upload", "mts_dev_download",
            "mts_net_create", "mts_net_destroy", "mts_net_param_count", "mts_net_flops", "mts_net_get_params", "mts_net_set_params",
            "mts_net_get_grads", "mts_net_forward", "mts_net_backward", "mts_net_adam", "mts_net_gemm_check",
@@ -434,6 +434,28 @@ class Batch:
     def set_state(self, g, f):
         f = np.ascontiguousarray(f, np.float64)
         _check(self.lib, self.lib.mts_batch_set_state(self.h, int(g), _p(f, _D)))
+
+    def set_states(self, items):
+        """Warm start of several lattices, collided in one launch: items is a
+        list of (g, populations)."""
+        fs = [np.ascontiguousarray(f, np.float64) for _, f in items]
+        gs = np.array([g for g, _ in items], np.int32)
+        ptrs = (_D * len(fs))(*[_p(f, _D) for f in fs])
+        self.lib.mts_batch_set_states.argtypes = [C.c_void_p, C.c_int, _I32, C.POINTER(_D)]
+        _check(self.lib, self.lib.mts_batch_set_states(self.h, len(fs), _p(gs, _I32), ptrs))
+
+    def seed(self, items, rho):
+        """Warm start from velocity fields on the device (mts_batch_seed):
+        items is a list of (g, ux, uy, tau) with ux, uy the lattice's
+        row-major velocity (zero in the solids)."""
+        us = [np.ascontiguousarray(u, np.float64) for _, u, _, _ in items]
+        vs = [np.ascontiguousarray(v, np.float64) for _, _, v, _ in items]
+        gs = np.array([it[0] for it in items], np.int32)
+        taus = np.array([it[3] for it in items], np.float64)
+        pu = (_D * len(us))(*[_p(u, _D) for u in us])
+        pv = (_D * len(vs))(*[_p(v, _D) for v in vs])
+        self.lib.mts_batch_seed.argtypes = [C.c_void_p, C.c_int, _I32, C.POINTER(_D), C.POINTER(_D), _D, C.c_double]
+        _check(self.lib, self.lib.mts_batch_seed(self.h, len(us), _p(gs, _I32), pu, pv, _p(taus, _D), float(rho)))
 
     def stream_handle(self):
         s = C.c_void_p()

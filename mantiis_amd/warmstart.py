@@ -74,13 +74,20 @@ def features(s, F):
     return x
 
 
-def solve(batch_samples, F, tol=1e-6, check=100, max_steps=400000, warm=None):
-    """Runs the batched solver; warm: list of populations (9, H, W) or None.
-    Returns the run record and the velocity fields."""
+def solve(batch_samples, F, tol=1e-6, check=100, max_steps=400000, warm=None, warm_u=None, rho_in=None):
+    """Runs the batched solver.  warm: list of populations (9, H, W) to start
+    from; warm_u: list of velocity fields (ux, uy) that the device turns into
+    the same populations as seed_populations (needs rho_in).  Returns the run
+    record and the velocity fields."""
     b = Batch([(s["pore"], s["knf"], s["lp"]) for s in batch_samples], collision=2, wall=2, dx=1e-9, Fbody=F)
     if warm is not None:
-        for g, f in enumerate(warm):
-            b.set_state(g, f)
+        b.set_states(list(enumerate(warm)))
+    if warm_u is not None:
+        items = []
+        for g, (s, (ux, uy)) in enumerate(zip(batch_samples, warm_u)):
+            fl = s["pore"] == 1
+            items.append((g, np.where(fl, ux, 0.0), np.where(fl, uy, 0.0), tau_of(s["kn"])))
+        b.seed(items, rho_in)
     r = b.run(max_steps, check_every=check, tol=tol)
     fields = [b.fields(g) for g in range(len(batch_samples))]
     b.close()

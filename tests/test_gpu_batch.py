@@ -52,6 +52,29 @@ def test_batch_lattices_equal_single_block(batch_steps):
             assert d <= 1e-8 and rel_l2(bux.reshape(-1)[fl], ux[fl]) < 1e-6
 
 
+def test_batch_beyond_4gib_of_populations():
+    """A batch whose population planes span more than 4 GiB (240 lattices of
+    1024^2, about 150 M cells, so plane offsets need 64 bits): its last
+    lattice equals the same lattice run alone, bit for bit."""
+    banner("batch of 240 x 1024^2 lattices (> 4 GiB of populations) vs the last lattice alone, 50 steps")
+    pore = G.disk_packing(1024, 0.6, 3, 16, seed=11)
+    k, lp = G.uniform_kn(pore, 0.2, MFP_DX)
+    n = 240
+    b = Batch([(pore, k, lp)] * n, **PARAMS)
+    total = int(b.cells().sum())
+    print("cells", total, "population bytes per buffer", 36 * total, flush=True)
+    assert 36 * total > 2 ** 32
+    b.run(50, check_every=50, tol=1e-300, graph=False)
+    last = b.fields(n - 1)
+    b.close()
+    c = Batch([(pore, k, lp)], **PARAMS)
+    c.run(50, check_every=50, tol=1e-300, graph=False)
+    one = c.fields(0)
+    c.close()
+    for x, y in zip(last, one):
+        assert np.array_equal(x, y)
+
+
 def test_batch_graph_replays_equal_eager():
     banner("batch graph replays vs eager launches, 6 chunks of 50 steps")
     gs = geoms()
@@ -99,6 +122,22 @@ def test_batch_warm_start_from_converged_state():
     r = c.run(200000, check_every=100, tol=1e-6)
     print("cold", cold, "warm", r["steps"])
     assert (r["steps"] > 0).all() and (r["steps"] < cold).all()
+    # seeding several lattices in one launch equals seeding them one by one
+    d = Batch(gs, **PARAMS)
+    e = Batch(gs, **PARAMS)
+    seeds = []
+    for g in range(len(gs)):
+        rho, ux, uy = b.fields(g)
+        cu = ex[:, None, None] * ux[None] + ey[:, None, None] * uy[None]
+        seeds.append(w[:, None, None] * rho[None] * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (ux ** 2 + uy ** 2)[None]))
+    for g, f in enumerate(seeds):
+        d.set_state(g, f)
+    e.set_states(list(enumerate(seeds)))
+    d.run(300, check_every=100, tol=1e-300, graph=False)
+    e.run(300, check_every=100, tol=1e-300, graph=False)
+    for g in range(len(gs)):
+        for x, y in zip(d.fields(g), e.fields(g)):
+            assert np.array_equal(x, y)
 
 
 @pytest.mark.parametrize("odd", [False, True])
@@ -215,3 +254,35 @@ def test_batch_mdbb_equals_single_block():
         d = max(np.abs(bux.reshape(-1) - ux)[fl].max(), np.abs(brho.reshape(-1) - rho)[fl].max())
         print("lattice", g, "max |diff|", d)
         assert d == 0.0
+
+
+@pytest.mark.parametrize("coll,wall", [(2, 0), (1, 2), (1, 0), (0, 2), (0, 0)],
+                         ids=["RMRT-BB", "MRT-SRBB", "MRT-BB", "SRT-SRBB", "SRT-BB"])
+def test_batch_operators_against_oracle(coll, wall):
+    """Every other collision / wall pair of the batched kernel: two lattices
+    of a batch against the FP64 oracle with the same operators after 100
+    steps (MRT without regularisation drifts on these lattices, so the pairs
+    are compared early), and against the single-block FP32 path bit for bit."""
+    import _oracle as O
+    banner(f"batch collision {coll} wall {wall} vs the oracle and the single-block path, 100 steps")
+    gs = geoms()[:2]
+    prm = dict(PARAMS, collision=coll, wall=wall)
+    b = Batch(gs, **prm)
+    b.run(100, check_every=100, tol=1e-300, graph=False)
+    for g, (pore, kn, lp) in enumerate(gs):
+        o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-6, periodic_y=True)
+        o.set_operators(coll, wall)
+        o.run("transport", 100, 0.0)
+        oux, ouy, orho = o.complete_domain()
+        rho, ux, uy = b.fields(g)
+        e = dict(rho=rel_l2(rho.reshape(-1), orho), ux=rel_l2(ux.reshape(-1), oux), uy=rel_l2(uy.reshape(-1), ouy))
+        print("lattice", g, "batch FP32 vs oracle FP64 rel-l2", e, flush=True)
+        assert e["rho"] < 1e-8 and e["ux"] < 1e-5 and e["uy"] < 1e-5, e
+        o.close()
+        lat = Lattice(pore, kn, lp, precision="fp32dev", periodic_y=1, **prm)
+        lat.step("transport", 100)
+        lux, luy, lrho = lat.complete_domain()
+        fl = pore.reshape(-1) == 1
+        assert np.array_equal(ux.reshape(-1)[fl], lux[fl]) and np.array_equal(rho.reshape(-1)[fl], lrho[fl])
+        lat.close()
+    b.close()

@@ -181,6 +181,29 @@ def test_nonfinite_state_is_reported(kind, prec):
     g.close()
 
 
+def test_absurd_allocation_reports_oom():
+    """A device allocation far beyond the GPU's memory (1 PiB) returns
+    MTS_ERR_OOM (-5) with the runtime's message, and the library keeps
+    working: a lattice created and stepped afterwards matches one stepped
+    before (no failed status left behind for a later launch check)."""
+    banner("OOM: 1 PiB device allocation, then a lattice")
+    from mantiis_amd import geometry as G
+    from mantiis_amd.hip import DeviceArray, MtsError
+    pore = G.disk_packing(64, 0.7, 3, 8, seed=2)
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    a = Lattice(pore, kn, lp, precision="fp32dev", periodic_y=1, dx=1e-9, Fbody=1e-6)
+    a.step("transport", 20)
+    with pytest.raises(MtsError) as e:
+        DeviceArray((1 << 48,), np.float32)
+    print("error", e.value)
+    assert e.value.code == -5 and "hipMalloc" in str(e.value)
+    b = Lattice(pore, kn, lp, precision="fp32dev", periodic_y=1, dx=1e-9, Fbody=1e-6)
+    b.step("transport", 20)
+    assert np.array_equal(a.get("f"), b.get("f"))
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("coll", [2, 1, 0], ids=["RMRT", "MRT", "SRT"])
 def test_fp32_mdbb_against_fp64_oracle(coll):
     """MDBB walls (coreLB.hpp:936-967) in the fused FP32 step: each wall cell

@@ -79,3 +79,23 @@ def test_warm_start_at_config_sizes():
     print("warm vs cold steady k", np.abs(kw / kc - 1).max(), flush=True)
     assert (rc["steps"] > 0).all() and (rw["steps"] > 0).all()
     assert np.abs(kw / kc - 1).max() < 0.02
+
+
+def test_device_seed_equals_host_seed():
+    """The warm-start populations formed on the device from a velocity field
+    (mts_batch_seed) equal seed_populations' host formulas: the same FP64
+    operations in the same order, so the runs that start from them agree bit
+    for bit."""
+    from mantiis_amd import warmstart as WS
+    from mantiis_amd.hip import globals_for
+    banner("warm start seeding on the device vs on the host, 3 lattices of 128^2")
+    rho_in = globals_for(dx=1e-9)[1]["rhoin"]
+    s_list = WS.samples(range(900, 903), 128, workers=2)
+    _, fields = WS.solve(s_list, F, tol=1e-3)
+    us = [(f[1] * 1.3, f[2] * 0.7) for f in fields]  # a velocity that is not the steady one
+    host = [WS.seed_populations(s, ux, uy, rho_in) for s, (ux, uy) in zip(s_list, us)]
+    ra, fa = WS.solve(s_list, F, warm=host, tol=1e-300, max_steps=200)
+    rb, fb = WS.solve(s_list, F, warm_u=us, rho_in=rho_in, tol=1e-300, max_steps=200)
+    for a, b in zip(fa, fb):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
