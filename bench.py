@@ -452,7 +452,7 @@ def bench_warmstart(args):
     t0 = time.perf_counter()
     train_s = WS.samples(range(10000, 10000 + args.train), 256)
     print(f"[bench] {len(train_s)} training lattices generated", file=sys.stderr, flush=True)
-    r, fields = WS.solve(train_s, F)
+    r, fields = WS.solve(train_s, F, graph=not args.eager)
     t_data = time.perf_counter() - t0
     print(f"[bench] training lattices solved to steady state ({t_data:.1f} s)", file=sys.stderr, flush=True)
     ds = WS.Dataset([WS.features(s, F) for s in train_s], [WS.targets(s, f, F) for s, f in zip(train_s, fields)])
@@ -471,7 +471,7 @@ def bench_warmstart(args):
     eval_s = WS.samples(range(20000, 20000 + args.eval), 512)
     t0 = time.perf_counter()
     print(f"[bench] {len(eval_s)} evaluation lattices generated", file=sys.stderr, flush=True)
-    rc, fc = WS.solve(eval_s, F)
+    rc, fc = WS.solve(eval_s, F, graph=not args.eager)
     print("[bench] evaluation lattices solved from rest", file=sys.stderr, flush=True)
     t_cold = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -480,7 +480,7 @@ def bench_warmstart(args):
     t0 = time.perf_counter()
     t_seed = 0.0  # the populations are formed on the device inside the warm-started solve (mts_batch_seed)
     print("[bench] predicted", file=sys.stderr, flush=True)
-    rw, fw = WS.solve(eval_s, F, warm_u=pred, rho_in=rho_in)
+    rw, fw = WS.solve(eval_s, F, warm_u=pred, rho_in=rho_in, graph=not args.eager)
     t_warm = time.perf_counter() - t0
     kc = np.array([WS.permeability(s, f[1], F) for s, f in zip(eval_s, fc)])
     kw = np.array([WS.permeability(s, f[1], F) for s, f in zip(eval_s, fw)])
@@ -600,8 +600,9 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="batched: number of lattices")
     ap.add_argument("--workers", type=int, default=8, help="batched/warmstart: processes generating the geometries")
     ap.add_argument("--eager", action="store_true",
-                    help="batched: launch the steps one by one instead of replaying a captured graph per chunk "
-                         "(rocprofv3's kernel trace fails inside HIP graph instantiation)")
+                    help="batched / warmstart: launch the steps one by one instead of replaying a captured graph "
+                         "per chunk (a rocprofv3 kernel trace of the graph path ends in a segmentation fault inside "
+                         "the HIP runtime, below mts_batch_s::chunk_graph)")
     ap.add_argument("--no-steady", action="store_true", help="batched / quadtree: skip the run to steady state")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU sample steps (default: quadtree 1000, shanchen 16)")

@@ -44,7 +44,7 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;
+constexpr int TM = 64, TN = 64, TK = 64, LDK = TK + 8;
 
 __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
@@ -53,6 +53,15 @@ __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >
 // Lane layout of v_mfma_f32_16x16x32_bf16 (wave64): lane l holds
 // A[l % 16][8 (l / 16) .. +7], B[8 (l / 16) .. +7][l % 16] and
 // D[4 (l / 16) .. +3][l % 16].
+// A stage is TK = 64 deep: each thread holds two 8-element fragments of A and
+// of B in registers.  The next stage's fragments are loaded from memory
+// before the current stage's eight MFMAs per wave, and written to LDS after
+// them, so the loads overlap the matrix work.
+typedef uint4 v16;
+struct Frag {
+    v16 v;
+};
+
 template <class Op>
 __global__ void __launch_bounds__(256) k_gemm(Op op, int M, int N, int K, int nsplit, int kchunk) {
     __shared__ bf16 sA[TM][LDK];
@@ -66,21 +75,37 @@ __global__ void __launch_bounds__(256) k_gemm(Op op, int M, int N, int K, int ns
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    Frag fa[2], fb[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        op.load_a(fa[h], m0, kb, ke, M, z, t, h);
+        op.load_b(fb[h], n0, kb, ke, N, z, t, h);
+    }
     for (int k0 = kb; k0 < ke; k0 += TK) {
-        op.fill_a(sA, m0, k0, ke, M, z, t);
-        op.fill_b(sB, n0, k0, ke, N, z, t);
+        op.store_a(sA, fa, t);
+        op.store_b(sB, fb, t);
         __syncthreads();
+        if (k0 + TK < ke) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                op.load_a(fa[h], m0, k0 + TK, ke, M, z, t, h);
+                op.load_b(fb[h], n0, k0 + TK, ke, N, z, t, h);
+            }
+        }
         const int i = l & 15, q = (l >> 4) * 8;
-        bf16x8 af[2], bfm[2];
 #pragma unroll
-        for (int a = 0; a < 2; a++) af[a] = *(const bf16x8*)&sA[wm + a * 16 + i][q];
+        for (int kh = 0; kh < 2; kh++) {
+            bf16x8 af[2], bfm[2];
 #pragma unroll
-        for (int b = 0; b < 2; b++) bfm[b] = *(const bf16x8*)&sB[wn + b * 16 + i][q];
+            for (int a = 0; a < 2; a++) af[a] = *(const bf16x8*)&sA[wm + a * 16 + i][32 * kh + q];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) bfm[b] = *(const bf16x8*)&sB[wn + b * 16 + i][32 * kh + q];
 #pragma unroll
-            for (int b = 0; b < 2; b++)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfm[b], acc[a][b], 0, 0, 0);
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfm[b], acc[a][b], 0, 0, 0);
+        }
         __syncthreads();
     }
 #pragma unroll
@@ -94,33 +119,56 @@ __global__ void __launch_bounds__(256) k_gemm(Op op, int M, int N, int K, int ns
             }
 }
 
-// LDS tile fills.  Row fill: thread t takes row t/4 and the 8 consecutive k
-// from 8 (t%4); one 16-byte load when the operand keeps those 8 k
-// contiguous (pa/pb), element loads otherwise.  Transposed fill (operands
-// whose 8 contiguous elements run along the tile's row index, e.g. 8
-// channels of one pixel): thread t takes rows 8 (t%8) .. +7 at k = t/8.
-typedef uint4 v16;
-
+// Fragment loads and LDS stores.  Row layout: thread t takes tile row t/4
+// and the 8 consecutive k from 8 (t%4) + 32 h; one 16-byte load when the
+// operand keeps those 8 k contiguous (pa/pb), element loads otherwise.
+// Transposed layout (operands whose 8 contiguous elements run along the
+// tile's row index, e.g. 8 channels of one pixel): thread t takes rows
+// 8 (t%8) .. +7 at k = 2 (t/8) + h, so its two fragments fill 4-byte pairs.
+__device__ __forceinline__ void row_store(bf16 (*s)[LDK], const Frag* f, int t) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) *(v16*)&s[t >> 2][(t & 3) * 8 + 32 * h] = f[h].v;
+}
+__device__ __forceinline__ void tr_store(bf16 (*s)[LDK], const Frag* f, int t) {
+    const uint16_t* a = (const uint16_t*)&f[0].v;
+    const uint16_t* b = (const uint16_t*)&f[1].v;
+    const int gq = (t & 7) * 8, p = 2 * (t >> 3);
+#pragma unroll
+    for (int e = 0; e < 8; e++) *(uint32_t*)&s[gq + e][p] = (uint32_t)a[e] | ((uint32_t)b[e] << 16);
+}
 template <class Op>
-__device__ __forceinline__ void fill_rows_a(const Op& op, bf16 (*s)[LDK], int m0, int k0, int ke, int M, int z, int t) {
-    const int r = t >> 2, kq = (t & 3) * 8, m = m0 + r, k = k0 + kq;
+__device__ __forceinline__ void row_load_a(const Op& op, Frag& f, int m0, int k0, int ke, int M, int z, int t, int h) {
+    const int m = m0 + (t >> 2), k = k0 + (t & 3) * 8 + 32 * h;
     if (op.vec_a && m < M && k + 7 < ke) {
-        *(v16*)&s[r][kq] = *(const v16*)op.pa(m, k, z);
+        f.v = *(const v16*)op.pa(m, k, z);
         return;
     }
+    bf16* e8 = (bf16*)&f.v;
 #pragma unroll
-    for (int e = 0; e < 8; e++) s[r][kq + e] = (m < M && k + e < ke) ? op.a(m, k + e, z) : (bf16)0.0f;
+    for (int e = 0; e < 8; e++) e8[e] = (m < M && k + e < ke) ? op.a(m, k + e, z) : (bf16)0.0f;
 }
 template <class Op>
-__device__ __forceinline__ void fill_rows_b(const Op& op, bf16 (*s)[LDK], int n0, int k0, int ke, int N, int z, int t) {
-    const int r = t >> 2, kq = (t & 3) * 8, n = n0 + r, k = k0 + kq;
+__device__ __forceinline__ void row_load_b(const Op& op, Frag& f, int n0, int k0, int ke, int N, int z, int t, int h) {
+    const int n = n0 + (t >> 2), k = k0 + (t & 3) * 8 + 32 * h;
     if (op.vec_b && n < N && k + 7 < ke) {
-        *(v16*)&s[r][kq] = *(const v16*)op.pb(k, n, z);
+        f.v = *(const v16*)op.pb(k, n, z);
         return;
     }
+    bf16* e8 = (bf16*)&f.v;
 #pragma unroll
-    for (int e = 0; e < 8; e++) s[r][kq + e] = (n < N && k + e < ke) ? op.b(k + e, n, z) : (bf16)0.0f;
+    for (int e = 0; e < 8; e++) e8[e] = (n < N && k + e < ke) ? op.b(k + e, n, z) : (bf16)0.0f;
 }
+
+// row-layout operands (dense and convolution forward / backward-data)
+#define MTS_ROW_FILLS                                                                                       \
+    __device__ void load_a(Frag& f, int m0, int k0, int ke, int M, int z, int t, int h) const {             \
+        row_load_a(*this, f, m0, k0, ke, M, z, t, h);                                                       \
+    }                                                                                                       \
+    __device__ void load_b(Frag& f, int n0, int k0, int ke, int N, int z, int t, int h) const {             \
+        row_load_b(*this, f, n0, k0, ke, N, z, t, h);                                                       \
+    }                                                                                                       \
+    __device__ void store_a(bf16 (*s)[LDK], const Frag* f, int t) const { row_store(s, f, t); }             \
+    __device__ void store_b(bf16 (*s)[LDK], const Frag* f, int t) const { row_store(s, f, t); }
 
 // dense row-major operands (the GEMM check)
 struct OpDense {
@@ -132,12 +180,7 @@ struct OpDense {
     __device__ bf16 b(int k, int n, int) const { return B[(int64_t)k * N + n]; }
     __device__ const bf16* pa(int, int, int) const { return nullptr; }
     __device__ const bf16* pb(int, int, int) const { return nullptr; }
-    __device__ void fill_a(bf16 (*s)[LDK], int m0, int k0, int ke, int M, int z, int t) const {
-        fill_rows_a(*this, s, m0, k0, ke, M, z, t);
-    }
-    __device__ void fill_b(bf16 (*s)[LDK], int n0, int k0, int ke, int N, int z, int t) const {
-        fill_rows_b(*this, s, n0, k0, ke, N, z, t);
-    }
+    MTS_ROW_FILLS
     __device__ void store(int m, int n, float v, int) const { C[(int64_t)m * N + n] = v; }
 };
 
@@ -180,12 +223,7 @@ struct OpConvFwd {
     }
     __device__ bf16 a(int m, int k, int z) const { return *pa(m, k, z); }
     __device__ bf16 b(int k, int n, int z) const { return *pb(k, n, z); }
-    __device__ void fill_a(bf16 (*s)[LDK], int m0, int k0, int ke, int M, int z, int t) const {
-        fill_rows_a(*this, s, m0, k0, ke, M, z, t);
-    }
-    __device__ void fill_b(bf16 (*s)[LDK], int n0, int k0, int ke, int N, int z, int t) const {
-        fill_rows_b(*this, s, n0, k0, ke, N, z, t);
-    }
+    MTS_ROW_FILLS
     __device__ void store(int m, int n, float v, int) const {
         v += bias[n];
         if (relu) v = fmaxf(v, 0.0f);
@@ -215,27 +253,25 @@ struct OpConvBwdData {
     }
     __device__ bf16 a(int m, int k, int z) const { return *pa(m, k, z); }
     __device__ bf16 b(int k, int n, int z) const { return *pb(k, n, z); }
-    __device__ void fill_a(bf16 (*s)[LDK], int m0, int k0, int ke, int M, int z, int t) const {
-        fill_rows_a(*this, s, m0, k0, ke, M, z, t);
-    }
-    __device__ void fill_b(bf16 (*s)[LDK], int n0, int k0, int ke, int N, int z, int t) const {
-        fill_rows_b(*this, s, n0, k0, ke, N, z, t);
-    }
+    MTS_ROW_FILLS
     __device__ void store(int m, int n, float v, int) const {
         float* o = dX + (int64_t)m * Cin + n;
         *o = add ? *o + v : v;
     }
 };
 
+// Weight gradient of a convolution as one GEMM per split of the pixels:
+// dW[kk][co][ci] = sum_p dY[p][co] X[p shifted by kk][ci], with the kernel
+// offsets kk along the columns (n = kk * Cin + ci), so a block's dY tile
+// serves every offset of its columns.
 struct OpConvBwdW {
     Geom g;
     const bf16* dY;  // [p][Cout]
     const bf16* X;   // [p][Cin]
-    float* part;     // [z][Cout][Cin], z = kk * nsplit + s
-    int Cin, Cout, ksz, nsplit;
+    float* part;     // [z][Cout][nkk * Cin], z = split of the pixels
+    int Cin, Cout, ksz, ncol;
     bool vec_a, vec_b;  // Cout % 8 == 0, Cin % 8 == 0
-    __device__ int64_t xrow(int k, int z) const {  // site of pixel k shifted by the offset of z's kk
-        const int kk = z / nsplit;
+    __device__ int64_t xrow(int k, int kk) const {  // site of pixel k shifted by offset kk
         int b, y, x, dy, dx;
         g.pix(k, b, y, x);
         koff(kk, ksz, dy, dx);
@@ -243,57 +279,104 @@ struct OpConvBwdW {
         return ((int64_t)b * g.H + yy) * g.W + xx;
     }
     __device__ bf16 a(int m, int k, int) const { return dY[(int64_t)k * Cout + m]; }
-    __device__ bf16 b(int k, int n, int z) const { return X[xrow(k, z) * Cin + n]; }
-    // transposed fills: thread t loads the 8 channels 8 (t%8).. of pixel k0 + t/8
-    __device__ void fill_a(bf16 (*s)[LDK], int m0, int k0, int ke, int M, int, int t) const {
-        const int gq = (t & 7) * 8, p = t >> 3, m = m0 + gq, k = k0 + p;
+    __device__ bf16 b(int k, int n, int) const {
+        const int kk = n / Cin;
+        return X[xrow(k, kk) * Cin + (n - kk * Cin)];
+    }
+    // transposed fragments: thread t loads the 8 channels 8 (t%8).. of pixel
+    // k0 + 2 (t/8) + h
+    __device__ void load_a(Frag& f, int m0, int k0, int ke, int M, int, int t, int h) const {
+        const int m = m0 + (t & 7) * 8, k = k0 + 2 * (t >> 3) + h;
         if (vec_a && m + 7 < M && k < ke) {
-            const v16 v = *(const v16*)(dY + (int64_t)k * Cout + m);
-            const bf16* e8 = (const bf16*)&v;
-#pragma unroll
-            for (int e = 0; e < 8; e++) s[gq + e][p] = e8[e];
+            f.v = *(const v16*)(dY + (int64_t)k * Cout + m);
             return;
         }
+        bf16* e8 = (bf16*)&f.v;
 #pragma unroll
-        for (int e = 0; e < 8; e++) s[gq + e][p] = (m + e < M && k < ke) ? dY[(int64_t)k * Cout + m + e] : (bf16)0.0f;
+        for (int e = 0; e < 8; e++) e8[e] = (m + e < M && k < ke) ? dY[(int64_t)k * Cout + m + e] : (bf16)0.0f;
     }
-    __device__ void fill_b(bf16 (*s)[LDK], int n0, int k0, int ke, int N, int z, int t) const {
-        const int gq = (t & 7) * 8, p = t >> 3, n = n0 + gq, k = k0 + p;
+    __device__ void load_b(Frag& f, int n0, int k0, int ke, int N, int z, int t, int h) const {
+        const int n = n0 + (t & 7) * 8, k = k0 + 2 * (t >> 3) + h;
+        bf16* e8 = (bf16*)&f.v;
         if (k >= ke) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) s[gq + e][p] = (bf16)0.0f;
+            for (int e = 0; e < 8; e++) e8[e] = (bf16)0.0f;
             return;
         }
-        const bf16* row = X + xrow(k, z) * Cin;
-        if (vec_b && n + 7 < N) {
-            const v16 v = *(const v16*)(row + n);
-            const bf16* e8 = (const bf16*)&v;
-#pragma unroll
-            for (int e = 0; e < 8; e++) s[gq + e][p] = e8[e];
+        if (vec_b && n + 7 < N) {  // 8 channels of one offset (Cin % 8 == 0)
+            const int kk = n / Cin;
+            f.v = *(const v16*)(X + xrow(k, kk) * Cin + (n - kk * Cin));
             return;
         }
 #pragma unroll
-        for (int e = 0; e < 8; e++) s[gq + e][p] = n + e < N ? row[n + e] : (bf16)0.0f;
+        for (int e = 0; e < 8; e++) e8[e] = n + e < N ? b(k, n + e, z) : (bf16)0.0f;
     }
-    __device__ void store(int m, int n, float v, int z) const { part[((int64_t)z * Cout + m) * Cin + n] = v; }
+    __device__ void store_a(bf16 (*s)[LDK], const Frag* f, int t) const { tr_store(s, f, t); }
+    __device__ void store_b(bf16 (*s)[LDK], const Frag* f, int t) const { tr_store(s, f, t); }
+    __device__ void store(int m, int n, float v, int z) const { part[((int64_t)z * Cout + m) * ncol + n] = v; }
 };
 
+// pixels per split of the weight-gradient GEMM (at most BW_SPLITS splits)
+constexpr int BW_PIX = 2048, BW_SPLITS = 256;
+inline int bw_nsplit(int64_t Pn) { return (int)std::max<int64_t>(1, std::min<int64_t>(BW_SPLITS, (Pn + BW_PIX - 1) / BW_PIX)); }
+
 // ---------------------------------------------------------------- small kernels
-__global__ void k_sum_parts(const float* part, float* dW, int nkk, int nsplit, int64_t per) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nkk * per) return;
-    const int64_t kk = i / per, e = i - kk * per;
-    float s = 0.0f;
-    for (int q = 0; q < nsplit; q++) s += part[((int64_t)kk * nsplit + q) * per + e];
-    dW[i] = s;
+// FP32 sum over the 64 lanes of a wave (butterfly: the same order on every
+// lane and every run)
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-// bias gradient: db[c] = sum_p g[p][c] (one block per channel, fixed order)
-__global__ void __launch_bounds__(256) k_bias_grad(const bf16* g, int64_t P, int C, float* db) {
+// dW[kk][co][ci] = the sum over the splits of part[z][co][kk Cin + ci]: one
+// wave per weight, lane l taking the splits l, l + 64, ..., then the wave's
+// butterfly
+__global__ void __launch_bounds__(256) k_sum_parts(const float* part, float* dW, int nkk, int Cout, int Cin,
+                                                   int nsplit) {
+    const int64_t per = (int64_t)Cout * Cin;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int l = threadIdx.x & 63;
+    if (i >= nkk * per) return;  // whole waves
+    const int kk = (int)(i / per);
+    const int64_t e = i - kk * per;
+    const int co = (int)(e / Cin), ci = (int)(e - (int64_t)co * Cin);
+    const int64_t ncol = (int64_t)nkk * Cin, off = (int64_t)co * ncol + (int64_t)kk * Cin + ci;
+    float s = 0.0f;
+    for (int q = l; q < nsplit; q += 64) s += part[(int64_t)q * Cout * ncol + off];
+    s = wave_sumf(s);
+    if (l == 0) dW[i] = s;
+}
+
+// bias gradient db[c] = sum_p g[p][c] in two passes with a fixed order:
+// per chunk of BG_CHUNK pixels, CW channels x 256/CW pixel lanes per block
+// (each pixel row of CW channels read as one run), the lanes combined in
+// order; then one block per channel adds the chunks by a fixed tree.
+constexpr int BG_CHUNK = 1024;
+template <int CW>
+__global__ void __launch_bounds__(256) k_bias_part(const bf16* g, int64_t P, int C, float* part) {
+    constexpr int PL = 256 / CW;
+    __shared__ float s[PL][CW];
+    const int cl = threadIdx.x % CW, pl = threadIdx.x / CW;
+    const int c = blockIdx.y * CW + cl;
+    const int64_t p0 = (int64_t)blockIdx.x * BG_CHUNK, p1 = min(P, p0 + BG_CHUNK);
+    float v = 0.0f;
+    if (c < C)
+        for (int64_t p = p0 + pl; p < p1; p += PL) v += (float)g[p * C + c];
+    s[pl][cl] = v;
+    __syncthreads();
+    if (pl == 0 && c < C) {
+        float a = 0.0f;
+#pragma unroll
+        for (int q = 0; q < PL; q++) a += s[q][cl];
+        part[(int64_t)blockIdx.x * C + c] = a;
+    }
+}
+__global__ void __launch_bounds__(256) k_bias_sum(const float* part, int nchunk, int C, float* db) {
     __shared__ float s[256];
     const int c = blockIdx.x, t = threadIdx.x;
     float v = 0.0f;
-    for (int64_t p = t; p < P; p += 256) v += (float)g[p * C + c];
+    for (int b = t; b < nchunk; b += 256) v += part[(int64_t)b * C + c];
     s[t] = v;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
@@ -382,13 +465,17 @@ __global__ void k_upcat_bwd_skip(const float* g, int C1, int C2, float* gskip, i
 }
 
 // masked mean-square loss over B*H*W pixels of 2 channels:
-// loss = sum m (o - t)^2 / (2 sum m); grad = m (o - t) / sum m
-__global__ void __launch_bounds__(256) k_loss(const float* o, const float* t, const float* mask, int64_t P, float* grad,
-                                              float* out2) {
+// loss = sum m (o - t)^2 / (2 sum m); grad = m (o - t) / sum m.
+// Pass 1: per chunk of LOSS_CHUNK pixels, (sum m (o - t)^2, sum m) by a
+// fixed tree; pass 2 (one thread): the chunks in order; pass 3: the gradient.
+constexpr int LOSS_CHUNK = 4096;
+__global__ void __launch_bounds__(256) k_loss_part(const float* o, const float* t, const float* mask, int64_t P,
+                                                   float* part) {
     __shared__ float s[2][256];
     const int tid = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * LOSS_CHUNK, p1 = min(P, p0 + LOSS_CHUNK);
     float se = 0.0f, sm = 0.0f;
-    for (int64_t p = tid; p < P; p += 256) {
+    for (int64_t p = p0 + tid; p < p1; p += 256) {
         const float m = mask[p];
         const float d0 = o[2 * p] - t[2 * p], d1 = o[2 * p + 1] - t[2 * p + 1];
         se += m * (d0 * d0 + d1 * d1);
@@ -404,16 +491,27 @@ __global__ void __launch_bounds__(256) k_loss(const float* o, const float* t, co
         }
         __syncthreads();
     }
-    const float denom = fmaxf(s[1][0], 1.0f);
-    for (int64_t p = tid; p < P; p += 256) {
-        const float m = mask[p] / denom;
-        grad[2 * p] = m * (o[2 * p] - t[2 * p]);
-        grad[2 * p + 1] = m * (o[2 * p + 1] - t[2 * p + 1]);
-    }
     if (tid == 0) {
-        out2[0] = s[0][0] / (2.0f * denom);
-        out2[1] = s[1][0];
+        part[2 * blockIdx.x] = s[0][0];
+        part[2 * blockIdx.x + 1] = s[1][0];
     }
+}
+__global__ void k_loss_sum(const float* part, int nchunk, float* out2) {
+    float se = 0.0f, sm = 0.0f;
+    for (int b = 0; b < nchunk; b++) {
+        se += part[2 * b];
+        sm += part[2 * b + 1];
+    }
+    out2[0] = se / (2.0f * fmaxf(sm, 1.0f));
+    out2[1] = sm;
+}
+__global__ void k_loss_grad(const float* o, const float* t, const float* mask, int64_t P, const float* out2,
+                            float* grad) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const float m = mask[p] / fmaxf(out2[1], 1.0f);
+    grad[2 * p] = m * (o[2 * p] - t[2 * p]);
+    grad[2 * p + 1] = m * (o[2 * p + 1] - t[2 * p + 1]);
 }
 
 __global__ void k_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2, float eps,
@@ -481,6 +579,8 @@ struct mts_net_s {
     float* part = nullptr;  // split-K partials
     int64_t part_n = 0;
     float* loss = nullptr;  // 2 floats
+    float* red = nullptr;   // partial sums of the bias gradients and the loss
+    int64_t red_n = 0;
     int B = 0, H = 0, W = 0;
     std::vector<void*> allocs;
     template <class Tp>
@@ -526,14 +626,17 @@ struct mts_net_s {
         const int64_t Pn = npix(in.level);
         const int64_t ny = Pn * l.cout;
         hipLaunchKernelGGL(k_grad_prep, dim3(nblk(ny)), dim3(256), 0, stream, gy, y, gb, ny);
-        hipLaunchKernelGGL(k_bias_grad, dim3(l.cout), dim3(256), 0, stream, gb, Pn, l.cout, G + l.boff);
+        const int nbc = (int)((Pn + BG_CHUNK - 1) / BG_CHUNK);
+        hipLaunchKernelGGL(k_bias_part<32>, dim3(nbc, (l.cout + 31) / 32), dim3(256), 0, stream, gb, Pn, l.cout, red);
+        hipLaunchKernelGGL(k_bias_sum, dim3(l.cout), dim3(256), 0, stream, red, nbc, l.cout, G + l.boff);
         const int nkk = l.ksz * l.ksz;
-        const int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, (Pn + 8191) / 8192));
+        const int nsplit = bw_nsplit(Pn);
         const int kchunk = (int)(((Pn + nsplit - 1) / nsplit + TK - 1) / TK * TK);
-        OpConvBwdW ow{geom(in.level), gb, in.a, part, l.cin, l.cout, l.ksz, nsplit, l.cout % 8 == 0, l.cin % 8 == 0};
-        gemm(ow, l.cout, l.cin, (int)Pn, nkk * nsplit, nsplit, kchunk, stream);
+        OpConvBwdW ow{geom(in.level), gb, in.a, part, l.cin, l.cout, l.ksz, nkk * l.cin, l.cout % 8 == 0, l.cin % 8 == 0};
+        gemm(ow, l.cout, nkk * l.cin, (int)Pn, nsplit, nsplit, kchunk, stream);
         const int64_t per = (int64_t)l.cout * l.cin;
-        hipLaunchKernelGGL(k_sum_parts, dim3(nblk(nkk * per)), dim3(256), 0, stream, part, G + l.woff, nkk, nsplit, per);
+        hipLaunchKernelGGL(k_sum_parts, dim3(nblk(nkk * per * 64)), dim3(256), 0, stream, part, G + l.woff, nkk,
+                           l.cout, l.cin, nsplit);
         if (gin) {
             OpConvBwdData od{geom(in.level), gb, PbT + l.woff, gin, l.cin, l.cout, l.ksz, add, l.cout % 8 == 0,
                              l.cout % 8 == 0};
@@ -545,6 +648,13 @@ struct mts_net_s {
 namespace {
 
 int check_net(mts_net n) { return n ? MTS_OK : set_error(MTS_ERR_ARG, "null network handle"); }
+
+// resolution level of each layer's input: two convolutions per encoder level
+// 0..3, two per decoder level 2..0, then the 1x1 head at level 0
+int layer_level(size_t i) {
+    static const int level[15] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+    return i < 15 ? level[i] : 0;
+}
 
 // layer table of the U-Net (widths c0..c3, input 3 channels, output 2)
 void layers(mts_net_s* n) {
@@ -614,13 +724,18 @@ int mts_net_create(int c0, int max_batch, int max_h, int max_w, uint64_t seed, m
     }
     const int64_t P0 = (int64_t)max_batch * max_h * max_w;
     int64_t pmax = 0;
-    for (const ConvL& l : n->L) {
-        const int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, (P0 + 8191) / 8192));
+    for (size_t i = 0; i < n->L.size(); i++) {  // split-K partials at each layer's resolution
+        const ConvL& l = n->L[i];
+        const int nsplit = bw_nsplit(P0 >> (2 * layer_level(i)));
         pmax = std::max(pmax, (int64_t)l.ksz * l.ksz * nsplit * l.cout * l.cin);
     }
     n->part_n = pmax;
+    // reduction partials: the bias gradient's chunks x channels, the loss's chunks x 2
+    int cmax = 2;
+    for (const ConvL& l : n->L) cmax = std::max(cmax, l.cout);
+    n->red_n = std::max((P0 + BG_CHUNK - 1) / BG_CHUNK * cmax, (P0 + LOSS_CHUNK - 1) / LOSS_CHUNK * 2);
     if ((rc = n->alloc(&n->out, P0 * 2)) || (rc = n->alloc(&n->gout, P0 * 2)) || (rc = n->alloc(&n->gb, maxe)) ||
-        (rc = n->alloc(&n->part, pmax)) || (rc = n->alloc(&n->loss, 2)))
+        (rc = n->alloc(&n->part, pmax)) || (rc = n->alloc(&n->loss, 2)) || (rc = n->alloc(&n->red, n->red_n)))
         return bail(rc);
     // He-normal weights, zero biases, zero Adam moments
     std::vector<float> h(np, 0.0f);
@@ -649,13 +764,11 @@ int mts_net_destroy(mts_net n) {
 int mts_net_flops(mts_net n, int B, int H, int W, double* fwd) {
     if (int rc = check_net(n)) return rc;
     if (!fwd || B <= 0 || H <= 0 || W <= 0) return set_error(MTS_ERR_ARG, "bad argument");
-    // resolution level of each layer's input: two convolutions per encoder
-    // level 0..3, two per decoder level 2..0, then the 1x1 head at level 0
-    static const int level[15] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
     double f = 0.0;
-    for (size_t i = 0; i < n->L.size() && i < 15; i++) {
+    for (size_t i = 0; i < n->L.size(); i++) {
         const ConvL& l = n->L[i];
-        f += 2.0 * B * (double)(H >> level[i]) * (double)(W >> level[i]) * l.cin * l.cout * l.ksz * l.ksz;
+        const int lv = layer_level(i);
+        f += 2.0 * B * (double)(H >> lv) * (double)(W >> lv) * l.cin * l.cout * l.ksz * l.ksz;
     }
     *fwd = f;
     return MTS_OK;
@@ -752,7 +865,10 @@ int mts_net_backward(mts_net n, const float* target, const float* mask, float* l
     typedef mts_net_s S;
     const int B = n->B, H = n->H, W = n->W;
     const int64_t p0 = n->npix(0);
-    hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, st, n->out, target, mask, p0, n->gout, n->loss);
+    const int nlc = (int)((p0 + LOSS_CHUNK - 1) / LOSS_CHUNK);
+    hipLaunchKernelGGL(k_loss_part, dim3(nlc), dim3(256), 0, st, n->out, target, mask, p0, n->red);
+    hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(1), 0, st, n->red, nlc, n->loss);
+    hipLaunchKernelGGL(k_loss_grad, dim3(nblk(p0)), dim3(256), 0, st, n->out, target, mask, p0, n->loss, n->gout);
     // head, then decoder, then encoder (reverse order of the forward pass)
     n->conv_bwd(14, T[S::D0], n->gout, nullptr, T[S::D0].g, 0);
     n->conv_bwd(13, T[S::D0A], T[S::D0].g, T[S::D0].a, T[S::D0A].g, 0);

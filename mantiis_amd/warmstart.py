@@ -74,7 +74,7 @@ def features(s, F):
     return x
 
 
-def solve(batch_samples, F, tol=1e-6, check=100, max_steps=400000, warm=None, warm_u=None, rho_in=None):
+def solve(batch_samples, F, tol=1e-6, check=100, max_steps=400000, warm=None, warm_u=None, rho_in=None, graph=True):
     """Runs the batched solver.  warm: list of populations (9, H, W) to start
     from; warm_u: list of velocity fields (ux, uy) that the device turns into
     the same populations as seed_populations (needs rho_in).  Returns the run
@@ -88,7 +88,7 @@ def solve(batch_samples, F, tol=1e-6, check=100, max_steps=400000, warm=None, wa
             fl = s["pore"] == 1
             items.append((g, np.where(fl, ux, 0.0), np.where(fl, uy, 0.0), tau_of(s["kn"])))
         b.seed(items, rho_in)
-    r = b.run(max_steps, check_every=check, tol=tol)
+    r = b.run(max_steps, check_every=check, tol=tol, graph=graph)
     fields = [b.fields(g) for g in range(len(batch_samples))]
     b.close()
     return r, fields
