@@ -67,7 +67,10 @@ typedef struct mts_params {
     int collision;   /* MTS_COLL_*                             default RMRT  */
     int wall;        /* MTS_WALL_*                             default SRBB  */
     int precision;   /* MTS_FP64 / MTS_FP32DEV                 default FP64  */
-    int multiblock;  /* 1: two-level quadtree grid (grid.hpp:313-399)      */
+    int multiblock;  /* 0: single block; 1: the reference's quadtree of
+                      * at most two levels (maxAllowedLevel = 2, grid.hpp:
+                      * 313-399, 1180); k = 2..MTS_MAX_LEVELS: at most k
+                      * levels (the same recursion with maxAllowedLevel = k) */
     int device;      /* HIP device ordinal                     default 0     */
     int periodic_y;  /* 0: the reference's seam rule (coreLB.hpp:301-331):
                       *    x-seam links periodic (overwrite after the wall
@@ -105,10 +108,13 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
                const mts_params* p, mts_sim* out);
 int mts_destroy(mts_sim s);
 
+/* Largest level count of a quadtree lattice (mts_params.multiblock). */
+#define MTS_MAX_LEVELS 3
+
 /* ---- Grid2D on the host (grid.hpp:167-1157), no device needed.
  * mts_grid_create builds the cell list of a row-major [y][x] geometry exactly
- * as Grid2D(shape) + Grid2D::initialize (single-block, or the two-level
- * quadtree when p->multiblock is set; p->periodic_y selects the seam rule).
+ * as Grid2D(shape) + Grid2D::initialize (single-block, or the quadtree when
+ * p->multiblock is set; p->periodic_y selects the seam rule).
  * Fails with MTS_ERR_GEOMETRY on an unsupported wall pattern. */
 typedef struct mts_grid_s* mts_grid;
 int mts_grid_create(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore,
@@ -125,8 +131,10 @@ int mts_grid_size(mts_grid g, int64_t* n, int32_t* max_level);
 int mts_grid_arrays(mts_grid g, int64_t* gridID, int32_t* gridIJ, int32_t* level, int64_t* connect, int32_t* bnd,
                     int32_t* type, int32_t* isBuffer, double* taus);
 /* which = 0: Grid2D::gridParentChild rows {cell, tl, tr, bl, br};
- * 1: coarseBuffer[21] rows; 2: fineBuffer[12] rows.  Rows of 5 int64;
- * call with rows = NULL to get *count (rows). */
+ * 1: the coarseBuffer rows of every pair of levels (coarseBuffer[21], then
+ * [32], ...); 2: the fineBuffer rows (fineBuffer[12], then [23], ...);
+ * which = a buffer type (21, 32, 12, 23): that type's rows only.  Rows of 5
+ * int64; call with rows = NULL to get *count (rows). */
 int mts_grid_rows(mts_grid g, int which, int64_t* rows, int64_t* count);
 
 /* ---- Quadtree tables on the device (no counterpart in the reference,

@@ -101,15 +101,19 @@ struct mts_sim_s {
     std::vector<LevelImage> limg;
     std::vector<LevelDev> lv;
     std::vector<CellParam> mparams;
+    // interface rows of every pair of levels (l, l + 1), one pair after the
+    // other: pair p = l - 1 owns rows [off[p], off[p + 1])
     CoarseRow* d_crows = nullptr;
     FineRow* d_frows = nullptr;
     int64_t ncrows = 0, nfrows = 0;
-    // FP32 interface plan (one launch per coarse step)
+    std::vector<int64_t> crow_off, frow_off;
+    // FP32 interface plans (one launch per pair of levels and coarse step),
+    // blocks of pair p: [ifb_off[p], ifb_off[p + 1])
     IfHead* d_ifhead = nullptr;
     IfTask* d_iftask = nullptr;
     IfCoarse* d_ifco = nullptr;
     IfFine* d_iffo = nullptr;
-    int nifb = 0;
+    std::vector<int> ifb_off;
     // FP32 multi-block state in cell order, per level: image (host and device,
     // with and without interface overrides), carried states, stored fields
     // (two sets: a step's interface reads the previous step's while the
@@ -319,13 +323,21 @@ static int upload_compact(mts_sim_s* s, const CompactImage& h, const float* cpar
 }
 
 // Interface rows of coreLB.hpp:1101-1231 with every coordinate and weight
-// input formed on the host as the reference forms them.
+// input formed on the host as the reference forms them, for each pair of
+// levels (l1, l2 = l1 + 1) in turn; off[p] are the first rows of pair p.
 static void interface_rows(const MultiGeometry& G, const std::vector<LevelImage>& L, std::vector<CoarseRow>& cr,
-                           std::vector<FineRow>& fr) {
+                           std::vector<FineRow>& fr, std::vector<int64_t>& coff, std::vector<int64_t>& foff) {
     auto site = [&](int64_t c) -> int64_t {
         const int l = G.level[c] - 1;
         return (int64_t)(G.cy[c] >> l) * L[l].nx + (G.cx[c] >> l);
     };
+    const int np = std::max(G.maxLevel - 1, 0);
+    coff.assign(np + 1, 0);
+    foff.assign(np + 1, 0);
+    for (int p = 0; p < np; p++) {
+        coff[p + 1] = coff[p] + G.coarseCount[p];
+        foff[p + 1] = foff[p] + G.fineCount[p];
+    }
     for (size_t r = 0; r + 4 < G.coarseRows.size() + 1; r += 5) {
         CoarseRow R{};
         const int64_t* row = &G.coarseRows[r];
@@ -346,13 +358,18 @@ static void interface_rows(const MultiGeometry& G, const std::vector<LevelImage>
     }
     static const int top[9] = {7, 4, 8, 3, 0, 1, 6, 2, 5};
     static const int bottom[9] = {6, 2, 5, 3, 0, 1, 7, 4, 8};
+    int p = 0;
     for (size_t r = 0; r + 4 < G.fineRows.size() + 1; r += 5) {
+        while (p + 1 < np && (int64_t)(r / 5) >= foff[p + 1]) p++;
+        const int l1 = p + 1, l2 = p + 2;
+        // centre shifts of the parent's and the children's level (coreLB.hpp:1134-1135)
+        const double pshift = std::pow(2.0, l2 - 2) - 0.5, cshift = l1 == 1 ? 0.0 : std::pow(2.0, l1 - 2) - 0.5;
         FineRow R{};
         const int64_t* row = &G.fineRows[r];
         const int64_t par = row[0];
         const int64_t* pc = &G.connect[9 * par];
         const int py = G.cy[par], px = G.cx[par];
-        const double pshift = std::pow(2.0, 2 - 2) - 0.5, half = std::pow(2, G.level[par] - 1);
+        const double half = std::pow(2, G.level[par] - 1);
         std::vector<double> X(9), Y(9);
         for (int ic = 0; ic < 9; ic++) {
             const int ny_ = pc[ic] >= 0 ? G.cy[pc[ic]] : py + 2 * EY[ic];
@@ -376,8 +393,8 @@ static void interface_rows(const MultiGeometry& G, const std::vector<LevelImage>
         for (int ch = 0; ch < 4; ch++) {
             const int64_t cc = row[1 + ch];
             R.child[ch] = site(cc);
-            R.qx[ch] = G.cx[cc] + 0.0 + 1;
-            R.qy[ch] = G.cy[cc] + 0.0 + 1;
+            R.qx[ch] = G.cx[cc] + cshift + 1;
+            R.qy[ch] = G.cy[cc] + cshift + 1;
             R.ratio[ch] = G.taus[cc] / (2 * G.taus[par]);
         }
         fr.push_back(R);
@@ -398,8 +415,10 @@ struct IfPlan {
     std::vector<IfFine> fo;
 };
 
+// fbase: the first override slot of the fine rows on the fine level (its
+// coarse-buffer slots, as the coarse level of the pair below, come first).
 static void build_if_plan(const std::vector<CoarseRow32>& cr, const std::vector<FineRow32>& fr, const CompactImage& h1,
-                          const CompactImage& h2, IfPlan& P) {
+                          const CompactImage& h2, int64_t fbase, IfPlan& P) {
     std::unordered_map<int32_t, int32_t> crowOf;  // coarse-buffer cell -> coarse row
     for (size_t r = 0; r < cr.size(); r++)
         if (cr[r].nch > 0) crowOf[cr[r].parent] = (int32_t)r;
@@ -493,7 +512,7 @@ static void build_if_plan(const std::vector<CoarseRow32>& cr, const std::vector<
             node[k] = (int16_t)(it == crowOf.end() ? task(2, R.src[k]) : coarse(it->second));
         }
         for (int ch = 0; ch < 4; ch++) {
-            IfFine o{R.child[ch], (int32_t)(4 * r + ch), R.ratio[ch], {0, 0, 0, 0, 0, 0}, h1.pidx[R.child[ch]], 0,
+            IfFine o{R.child[ch], (int32_t)(fbase + 4 * r + ch), R.ratio[ch], {0, 0, 0, 0, 0, 0}, h1.pidx[R.child[ch]], 0,
                      {0, 0, 0, 0, 0, 0}};
             for (int k = 0; k < 6; k++) {
                 o.node[k] = node[k];
@@ -513,8 +532,8 @@ static void build_if_plan(const std::vector<CoarseRow32>& cr, const std::vector<
 
 // Bounds of every index the interface kernel takes from the plan, and each
 // override slot written exactly once; false names the first violation.
-static bool check_if_plan(const IfPlan& P, size_t ncr, const std::vector<CoarseRow32>& cr, size_t nfr, int64_t ncell1,
-                          int64_t ncell2, size_t nparam, std::string& err) {
+static bool check_if_plan(const IfPlan& P, size_t ncr, const std::vector<CoarseRow32>& cr, size_t nfr, int64_t fbase,
+                          int64_t ncell1, int64_t ncell2, size_t nparam, std::string& err) {
     std::vector<int> wrote2(ncr, 0), wrote1(4 * nfr, 0);
     for (size_t b = 0; b < P.head.size(); b++) {
         const IfHead& H = P.head[b];
@@ -550,11 +569,12 @@ static bool check_if_plan(const IfPlan& P, size_t ncr, const std::vector<CoarseR
                     err = "interface plan: node slot";
                     return false;
                 }
-            if (F.child < 0 || F.child >= ncell1 || F.out < 0 || F.out >= (int64_t)(4 * nfr) || F.pidx >= nparam) {
+            if (F.child < 0 || F.child >= ncell1 || F.out < fbase || F.out >= fbase + (int64_t)(4 * nfr) ||
+                F.pidx >= nparam) {
                 err = "interface plan: fine record";
                 return false;
             }
-            wrote1[F.out]++;
+            wrote1[F.out - fbase]++;
         }
     }
     for (size_t r = 0; r < ncr; r++)
@@ -571,30 +591,36 @@ static bool check_if_plan(const IfPlan& P, size_t ncr, const std::vector<CoarseR
 }
 
 // Cell-order images of the levels for the FP32 path.  Buffer cells written
-// by the interface get an override slot: coarse buffer of row r -> r, fine
-// buffer k of row r -> 4 r + k.
+// by the interface get an override slot on their level: the coarse buffer
+// of row r of the pair below -> r, then fine buffer k of row r of the pair
+// above -> (coarse rows of the pair below) + 4 r + k.
 static int create_multi32(mts_sim_s* s, const std::vector<CoarseRow>& cr, const std::vector<FineRow>& fr,
                           std::string& err) {
-    const int Lmax = (int)s->limg.size();
+    const int Lmax = (int)s->limg.size(), np = Lmax - 1;
+    const std::vector<int64_t>&co = s->crow_off, &fo = s->frow_off;
     // every interface site must be a cell of its level image
     auto is_cell = [&](int l, int64_t site) {
         const LevelImage& I = s->limg[l];
         return site >= 0 && site < (int64_t)I.nx * I.ny && (I.meta[site] & FLUID_BIT);
     };
-    if (Lmax > 1) {
-        for (const CoarseRow& R : cr) {
-            if (R.nch > 0 && !is_cell(1, R.parent)) return fail(MTS_ERR_GEOMETRY, "interface row: parent is not a cell");
+    for (int p = 0; p < np; p++) {
+        for (int64_t r = co[p]; r < co[p + 1]; r++) {
+            const CoarseRow& R = cr[r];
+            if (R.nch > 0 && !is_cell(p + 1, R.parent)) return fail(MTS_ERR_GEOMETRY, "interface row: parent is not a cell");
             for (int k = 0; k < 4; k++)
-                if (R.child[k] >= 0 && !is_cell(0, R.child[k]))
+                if (R.child[k] >= 0 && !is_cell(p, R.child[k]))
                     return fail(MTS_ERR_GEOMETRY, "interface row: child is not a cell");
         }
-        for (const FineRow& R : fr) {
+        for (int64_t r = fo[p]; r < fo[p + 1]; r++) {
+            const FineRow& R = fr[r];
             for (int k = 0; k < 6; k++)
-                if (!is_cell(1, R.src[k])) return fail(MTS_ERR_GEOMETRY, "interface row: node is not a cell");
+                if (!is_cell(p + 1, R.src[k])) return fail(MTS_ERR_GEOMETRY, "interface row: node is not a cell");
             for (int k = 0; k < 4; k++)
-                if (!is_cell(0, R.child[k])) return fail(MTS_ERR_GEOMETRY, "interface row: buffer is not a cell");
+                if (!is_cell(p, R.child[k])) return fail(MTS_ERR_GEOMETRY, "interface row: buffer is not a cell");
         }
     }
+    // first fine-buffer slot of level l (after the coarse buffers of pair l - 1)
+    auto fbase = [&](int l) -> int64_t { return l >= 1 ? co[l] - co[l - 1] : 0; };
     s->lc.resize(Lmax);
     const float* cpar;
     int rc;
@@ -604,27 +630,26 @@ static int create_multi32(mts_sim_s* s, const std::vector<CoarseRow>& cr, const 
         auto& C = s->lc[l];
         const int64_t P = (int64_t)I.nx * I.ny;
         std::vector<int32_t> ovr(P, -1);
-        int64_t nslot = 0;
-        if (l == 1) {
-            for (size_t r = 0; r < cr.size(); r++)
+        int64_t nslot = fbase(l);
+        if (l >= 1)
+            for (int64_t r = co[l - 1]; r < co[l]; r++)
                 if (cr[r].nch > 0) {
                     if (ovr[cr[r].parent] >= 0) {
                         err = "coarse buffer cell in two interface rows";
                         return MTS_ERR_GEOMETRY;
                     }
-                    ovr[cr[r].parent] = (int32_t)r;
+                    ovr[cr[r].parent] = (int32_t)(r - co[l - 1]);
                 }
-            nslot = (int64_t)cr.size();
-        } else if (l == 0 && Lmax > 1) {
-            for (size_t r = 0; r < fr.size(); r++)
+        if (l < np) {
+            for (int64_t r = fo[l]; r < fo[l + 1]; r++)
                 for (int k = 0; k < 4; k++) {
                     if (ovr[fr[r].child[k]] >= 0) {
                         err = "fine buffer cell in two interface rows";
                         return MTS_ERR_GEOMETRY;
                     }
-                    ovr[fr[r].child[k]] = (int32_t)(4 * r + k);
+                    ovr[fr[r].child[k]] = (int32_t)(fbase(l) + 4 * (r - fo[l]) + k);
                 }
-            nslot = 4 * (int64_t)fr.size();
+            nslot += 4 * (fo[l + 1] - fo[l]);
         }
         LatticeView v{I.nx, I.ny, s->mg.torus, I.meta.data(), I.pidx.data(), -1, -1, ovr.data()};
         build_compact(v, C.h);
@@ -641,13 +666,15 @@ static int create_multi32(mts_sim_s* s, const std::vector<CoarseRow>& cr, const 
         C.ovr.ovr = 1;
     }
     upload_spec(s->lc[0].h.spec, s->stream);
-    // interface rows in cell indices with host-formed weights
-    if (Lmax > 1) {
-        const auto& ci1 = s->lc[0].h.cidx;
-        const auto& ci2 = s->lc[1].h.cidx;
-        std::vector<CoarseRow32> c32(cr.size());
-        for (size_t r = 0; r < cr.size(); r++) {
-            const CoarseRow& R = cr[r];
+    // interface plans of the pairs, in cell indices with host-formed weights
+    IfPlan all;
+    s->ifb_off.assign(1, 0);
+    for (int p = 0; p < np; p++) {
+        const auto& ci1 = s->lc[p].h.cidx;
+        const auto& ci2 = s->lc[p + 1].h.cidx;
+        std::vector<CoarseRow32> c32(co[p + 1] - co[p]);
+        for (size_t r = 0; r < c32.size(); r++) {
+            const CoarseRow& R = cr[co[p] + r];
             CoarseRow32& o = c32[r];
             o.parent = R.nch > 0 ? ci2[R.parent] : 0;
             for (int k = 0; k < 4; k++) o.child[k] = R.child[k] >= 0 ? ci1[R.child[k]] : -1;
@@ -655,9 +682,9 @@ static int create_multi32(mts_sim_s* s, const std::vector<CoarseRow>& cr, const 
             o.nch = R.nch;
             o.slot = (int32_t)r;
         }
-        std::vector<FineRow32> f32(fr.size());
-        for (size_t r = 0; r < fr.size(); r++) {
-            const FineRow& R = fr[r];
+        std::vector<FineRow32> f32(fo[p + 1] - fo[p]);
+        for (size_t r = 0; r < f32.size(); r++) {
+            const FineRow& R = fr[fo[p] + r];
             FineRow32& o = f32[r];
             for (int k = 0; k < 4; k++) o.child[k] = ci1[R.child[k]];
             for (int k = 0; k < 6; k++) o.src[k] = ci2[R.src[k]];
@@ -676,19 +703,25 @@ static int create_multi32(mts_sim_s* s, const std::vector<CoarseRow>& cr, const 
             }
         }
         IfPlan P;
-        build_if_plan(c32, f32, s->lc[0].h, s->lc[1].h, P);
-        if (!check_if_plan(P, c32.size(), c32, f32.size(), s->lc[0].h.ncell, s->lc[1].h.ncell, s->mparams.size(), err))
+        build_if_plan(c32, f32, s->lc[p].h, s->lc[p + 1].h, fbase(p), P);
+        if (!check_if_plan(P, c32.size(), c32, f32.size(), fbase(p), s->lc[p].h.ncell, s->lc[p + 1].h.ncell,
+                           s->mparams.size(), err))
             return fail(MTS_ERR_GEOMETRY, err);
-        s->nifb = (int)P.head.size();
-        if (s->nifb > 0 &&
-            ((rc = s->alloc((char**)&s->d_ifhead, P.head.size() * sizeof(IfHead))) ||
-             (rc = s->alloc((char**)&s->d_iftask, P.task.size() * sizeof(IfTask))) ||
-             (rc = s->alloc((char**)&s->d_ifco, P.co.size() * sizeof(IfCoarse))) ||
-             (rc = s->alloc((char**)&s->d_iffo, P.fo.size() * sizeof(IfFine))) ||
-             (rc = upload(s->d_ifhead, P.head.data(), P.head.size() * sizeof(IfHead), s->stream)) ||
-             (rc = upload(s->d_iftask, P.task.data(), P.task.size() * sizeof(IfTask), s->stream)) ||
-             (rc = upload(s->d_ifco, P.co.data(), P.co.size() * sizeof(IfCoarse), s->stream)) ||
-             (rc = upload(s->d_iffo, P.fo.data(), P.fo.size() * sizeof(IfFine), s->stream))))
+        all.head.insert(all.head.end(), P.head.begin(), P.head.end());
+        all.task.insert(all.task.end(), P.task.begin(), P.task.end());
+        all.co.insert(all.co.end(), P.co.begin(), P.co.end());
+        all.fo.insert(all.fo.end(), P.fo.begin(), P.fo.end());
+        s->ifb_off.push_back((int)all.head.size());
+    }
+    if (!all.head.empty()) {
+        if ((rc = s->alloc((char**)&s->d_ifhead, all.head.size() * sizeof(IfHead))) ||
+            (rc = s->alloc((char**)&s->d_iftask, all.task.size() * sizeof(IfTask))) ||
+            (rc = s->alloc((char**)&s->d_ifco, all.co.size() * sizeof(IfCoarse))) ||
+            (rc = s->alloc((char**)&s->d_iffo, all.fo.size() * sizeof(IfFine))) ||
+            (rc = upload(s->d_ifhead, all.head.data(), all.head.size() * sizeof(IfHead), s->stream)) ||
+            (rc = upload(s->d_iftask, all.task.data(), all.task.size() * sizeof(IfTask), s->stream)) ||
+            (rc = upload(s->d_ifco, all.co.data(), all.co.size() * sizeof(IfCoarse), s->stream)) ||
+            (rc = upload(s->d_iffo, all.fo.data(), all.fo.size() * sizeof(IfFine), s->stream)))
             return rc;
         HIP_TRY(hipStreamSynchronize(s->stream));  // the host plan goes out of scope
     }
@@ -706,14 +739,15 @@ static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const
     // cells (a sequential rule) and the parameter table on the host
     (void)locpore;
     const bool torus = s->prm.periodic_y != 0;
-    std::vector<uint8_t> coarse;
-    int rc = multi_check(nx, ny, pore, err);
+    const int K = quadtree_levels(s->prm.multiblock);
+    std::vector<uint8_t> lvl;
+    int rc = multi_check(nx, ny, pore, K, err);
     if (rc) return rc;
-    if ((rc = multi_coarse_device(nx, ny, pore, torus, s->stream, coarse, err))) return rc;
-    if ((rc = multi_cells(nx, ny, pore, kn, torus, coarse, s->mg, err))) return rc;
+    if ((rc = multi_levels_device(nx, ny, pore, torus, K, s->stream, lvl, err))) return rc;
+    if ((rc = multi_cells(nx, ny, pore, kn, torus, lvl, s->mg, err))) return rc;
     if ((rc = multi_grid_taus(s->ph, s->mg, err))) return rc;
     if ((rc = build_level_images(s->mg, s->limg, s->mparams, s->ph, err, false))) return rc;
-    if ((rc = multi_tables_device(s->mg, coarse, &s->limg, s->stream, err))) return rc;
+    if ((rc = multi_tables_device(s->mg, &s->limg, s->stream, err))) return rc;
     const size_t esz = s->prm.precision == MTS_FP32DEV ? 4 : 8;
     int64_t Pmax = 0;
     s->geo.nx = nx;
@@ -749,7 +783,7 @@ static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const
     if ((rc = upload(s->d_params, s->mparams.data(), s->mparams.size() * sizeof(CellParam), s->stream))) return rc;
     std::vector<CoarseRow> cr;
     std::vector<FineRow> fr;
-    interface_rows(s->mg, s->limg, cr, fr);
+    interface_rows(s->mg, s->limg, cr, fr, s->crow_off, s->frow_off);
     s->ncrows = (int64_t)cr.size();
     s->nfrows = (int64_t)fr.size();
     if ((rc = s->alloc((char**)&s->d_crows, cr.size() * sizeof(CoarseRow))) ||
@@ -771,31 +805,37 @@ static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const
 
 // One call = nsteps coarse steps of the transport loop on the quadtree
 // (mantiisApp.hpp:239-246 with evolutionStepOfMultiBlock, coreLB.hpp:1310-1331).
-// FP32 quadtree step in cell order: per coarse step the coarse level does one
-// fused substep, the fine level two (the second with the stored fields of
-// the first), then the interface forms the buffer cells' post-stream states.
-// From the second step of a call on, the interface of the previous step runs
-// in the launch of the step's first substeps and updates the buffer cells
-// itself; the last step's interface writes the overrides.  A call starts
-// from and ends in the canonical FP64 level images.
+// FP32 quadtree step in cell order: per coarse step level l does 2^(L-l)
+// fused substeps (all but the first with the stored fields of the first),
+// then the interface forms the buffer cells' post-stream states.  On two
+// levels, from the second step of a call on, the interface of the previous
+// step runs in the launch of the step's first substeps and updates the
+// buffer cells itself, and the last step's interface writes the overrides;
+// on three, each pair's interface writes overrides after every step.  A call
+// starts from and ends in the canonical FP64 level images.
 static int multi_transport32(mts_sim_s* s, int nsteps) {
     const int Lmax = (int)s->lv.size();
     int rc;
-    int cur[2] = {0, 0};
+    std::vector<int> cur(Lmax, 0);
     int mi = 0;  // stored-field set of the current step
-    auto iface = [&](int mprev) {
+    // the interface of the pair of levels (p + 1, p + 2)
+    auto iface = [&](int p, int mprev) {
         IfSpec f;
-        f.A1 = s->lc[0].A[cur[0]];
-        f.A2 = s->lc[1].A[cur[1]];
-        f.m1 = s->lc[0].m[mprev];
-        f.m2 = s->lc[1].m[mprev];
-        f.O1 = s->lc[0].O;
-        f.O2 = s->lc[1].O;
-        f.head = s->d_ifhead;
-        f.task = s->d_iftask;
-        f.co = s->d_ifco;
-        f.fo = s->d_iffo;
-        f.nblocks = s->nifb;
+        f.A1 = s->lc[p].A[cur[p]];
+        f.A2 = s->lc[p + 1].A[cur[p + 1]];
+        f.m1 = s->lc[p].m[mprev];
+        f.m2 = s->lc[p + 1].m[mprev];
+        f.O1 = s->lc[p].O;
+        f.O2 = s->lc[p + 1].O;
+        const int b0 = s->ifb_off[p];
+        f.head = s->d_ifhead + b0;
+        f.task = s->d_iftask + (int64_t)b0 * IF_TASKS;
+        f.co = s->d_ifco + (int64_t)b0 * IF_CO;
+        f.fo = s->d_iffo + (int64_t)b0 * IF_FO;
+        f.nblocks = s->ifb_off[p + 1] - b0;
+        // above the finest pair, children may be coarse buffers of the pair
+        // below, whose values that pair's interface has just written
+        f.child_ovr = p > 0 ? 1 : 0;
         return f;
     };
     for (int step = 0; step < nsteps; step++) {
@@ -804,7 +844,7 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
             // both levels' first substeps in one launch (with the previous
             // step's interface after the first step), then the second
             // level-1 substep with the stored fields
-            const IfSpec f = iface(mi ^ 1);
+            const IfSpec f = iface(0, mi ^ 1);
             MbSpec a[2];
             LaunchCtx c[2];
             for (int l = 0; l < 2; l++) {
@@ -835,7 +875,10 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
             if ((rc = s->launched())) return rc;
             continue;
         }
-        for (int l = Lmax == 2 ? -1 : Lmax - 1; l >= 0; l--) {
+        // three levels or more: each level's substeps, coarsest first, then
+        // the interface of each pair of levels, finest first
+        // (coreLB.hpp:1310-1331), written as overrides
+        for (int l = Lmax - 1; l >= 0; l--) {
             auto& C = s->lc[l];
             auto& L = s->lv[l];
             LaunchCtx c = s->lctx(l);
@@ -858,13 +901,19 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
                 cur[l] ^= 1;
             }
         }
+        for (int p = 0; p + 1 < Lmax; p++) {
+            LaunchCtx c1 = s->lctx(p), c2 = s->lctx(p + 1);
+            c1.cimg = &s->lc[p].ovr;
+            c2.cimg = &s->lc[p + 1].ovr;
+            if ((rc = launch_cinterface(c1, c2, iface(p, 0)))) return fail(rc, "interface has no kernel");
+        }
         if ((rc = s->launched())) return rc;
     }
     if (Lmax == 2) {  // the last step's interface, as overrides for the export
         LaunchCtx c1 = s->lctx(0), c2 = s->lctx(1);
         c1.cimg = &s->lc[0].ovr;
         c2.cimg = &s->lc[1].ovr;
-        if ((rc = launch_cinterface(c1, c2, iface(mi ^ 1)))) return fail(rc, "interface has no kernel");
+        if ((rc = launch_cinterface(c1, c2, iface(0, mi ^ 1)))) return fail(rc, "interface has no kernel");
     }
     // export: post-stream state with the interface overrides -> canonical
     for (int l = 0; l < Lmax; l++) {
@@ -923,12 +972,13 @@ static int multi_transport(mts_sim_s* s, int nsteps) {
                 launch_transport(c, 2, b);
             }
         }
-        if (Lmax > 1) {
-            auto &L1 = s->lv[0], &L2 = s->lv[1];
-            launch_coarse_interp(s->stream, s->ncrows, s->d_crows, L2.d_f, L2.stride, L1.d_f, L1.stride, L2.d_rho, L2.d_ux,
-                                 L2.d_uy, L2.d_usq);
-            launch_fine_interp(s->stream, s->nfrows, s->d_frows, L1.d_f, L1.stride, L2.d_f, L2.stride, L1.d_rho, L1.d_ux,
-                               L1.d_uy, L1.d_usq);
+        // interpolateBlockInterface(l, l + 1) for l = 1 .. Lmax - 1 in turn
+        for (int p = 0; p + 1 < Lmax; p++) {
+            auto &L1 = s->lv[p], &L2 = s->lv[p + 1];
+            launch_coarse_interp(s->stream, s->crow_off[p + 1] - s->crow_off[p], s->d_crows + s->crow_off[p], L2.d_f,
+                                 L2.stride, L1.d_f, L1.stride, L2.d_rho, L2.d_ux, L2.d_uy, L2.d_usq);
+            launch_fine_interp(s->stream, s->frow_off[p + 1] - s->frow_off[p], s->d_frows + s->frow_off[p], L1.d_f,
+                               L1.stride, L2.d_f, L2.stride, L1.d_rho, L1.d_ux, L1.d_uy, L1.d_usq);
         }
         for (int l = 0; l < Lmax; l++) {
             auto& L = s->lv[l];
@@ -1139,7 +1189,9 @@ struct mts_grid_s {
 static int build_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore, const mts_params& prm,
                       const Physics& ph, mts_grid_s& g, std::string& err) {
     g.multi = prm.multiblock != 0;
-    if (g.multi) return build_multi_grid(nx, ny, pore, kn, locpore, ph, prm.periodic_y != 0, g.mg, err);
+    if (g.multi)
+        return build_multi_grid(nx, ny, pore, kn, locpore, ph, prm.periodic_y != 0, quadtree_levels(prm.multiblock), g.mg,
+                                err);
     return build_single_grid(nx, ny, pore, kn, locpore, ph, prm.periodic_y != 0, g.single, err);
 }
 
@@ -1223,11 +1275,30 @@ int mts_grid_arrays(mts_grid g, int64_t* gridID, int32_t* gridIJ, int32_t* level
 }
 
 int mts_grid_rows(mts_grid g, int which, int64_t* rows, int64_t* count) {
-    if (!g || !count || which < 0 || which > 2) return fail(MTS_ERR_ARG, "bad argument");
+    if (!g || !count) return fail(MTS_ERR_ARG, "null argument");
     static const std::vector<int64_t> none;
-    const std::vector<int64_t>& v = !g->multi ? none : which == 0 ? g->mg.parentChild : which == 1 ? g->mg.coarseRows : g->mg.fineRows;
-    *count = (int64_t)v.size() / 5;
-    if (rows) std::copy(v.begin(), v.end(), rows);
+    const MultiGeometry& G = g->mg;
+    const std::vector<int64_t>* v = &none;
+    int64_t r0 = 0, r1 = 0;
+    if (which >= 0 && which <= 2) {
+        v = !g->multi ? &none : which == 0 ? &G.parentChild : which == 1 ? &G.coarseRows : &G.fineRows;
+        r1 = (int64_t)v->size() / 5;
+    } else {
+        // one buffer type: coarse 10 (l + 1) + l or fine 10 l + l + 1
+        const int hi = which / 10, lo = which % 10;
+        const bool coarse = hi == lo + 1, fine = lo == hi + 1;
+        if (!(coarse || fine) || std::min(hi, lo) < 1 || std::max(hi, lo) > MTS_MAX_LEVELS)
+            return fail(MTS_ERR_ARG, "which: 0, 1, 2 or a buffer type (21, 32, 12, 23)");
+        const int p = std::min(hi, lo) - 1;
+        if (g->multi && p < G.maxLevel - 1) {
+            const std::vector<int64_t>& cnt = coarse ? G.coarseCount : G.fineCount;
+            v = coarse ? &G.coarseRows : &G.fineRows;
+            for (int q = 0; q < p; q++) r0 += cnt[q];
+            r1 = r0 + cnt[p];
+        }
+    }
+    *count = r1 - r0;
+    if (rows) std::copy(v->begin() + 5 * r0, v->begin() + 5 * r1, rows);
     return MTS_OK;
 }
 

@@ -136,6 +136,7 @@ struct IfSpec {
     const IfCoarse* co = nullptr;   // [nblocks * IF_CO]
     const IfFine* fo = nullptr;     // [nblocks * IF_FO]
     int nblocks = 0;
+    int child_ovr = 0;              // 1: a child with an override takes it
 };
 // Interface of one coarse step fused with the next step's first substeps
 // (launch_cmultistep2 with the overrides formed in the launch).

@@ -1,15 +1,18 @@
-// Host-side set-up of a two-level quadtree lattice (the multi-block grid of
-// grid.hpp:313-1157): wall distance, coarse/fine cell selection, buffer
-// cells, connectivity, parent/child and interface tables, and the per-level
-// site images with link words that the HIP kernels decode.  Plain C++.
+// Host-side set-up of a quadtree lattice (the multi-block grid of
+// grid.hpp:313-1157): wall distance, level decision, buffer cells,
+// connectivity, parent/child and interface tables, and the per-level site
+// images with link words that the HIP kernels decode.  Plain C++.
 //
-// Coarse cells are the aligned 2x2 fluid blocks whose four pixels are at a
-// Euclidean distance >= 6 from every wall pixel (the quadtree split rule of
-// grid.hpp:1180-1214 with maxAllowedLevel = 2).  Cells are numbered in
-// row-major order of their top-left pixel, followed by the buffer cells in
-// creation order (grid.hpp:808-888).  On a torus (periodic_y = 1) the
-// distance transform and the buffer rule wrap around both seams; otherwise
-// they stop at the image edge as the reference's do.
+// With at most K levels, a cell of level l >= 2 is an aligned block of side
+// h = 2^(l-1) whose pixels are fluid at a Euclidean distance of at least
+// ceil(2^(K-1) sqrt 2) h from every wall pixel, and no larger block holding
+// it is one (the quadtree split rule of grid.hpp:1180-1214 with
+// maxAllowedLevel = K; the reference fixes K = 2: blocks of 2 x 2 at
+// distance >= 6).  Cells are numbered in row-major order of their top-left
+// pixel, followed by the buffer cells in creation order (grid.hpp:808-888).
+// On a torus (periodic_y = 1) the distance transform and the buffer rule wrap
+// around both seams; otherwise they stop at the image edge as the
+// reference's do.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -95,7 +98,11 @@ int wrapc(int v, int n) { return ((v % n) + n) % n; }
 
 }  // namespace
 
-int multi_check(int nx, int ny, const int32_t* pore, std::string& err) {
+int multi_check(int nx, int ny, const int32_t* pore, int K, std::string& err) {
+    if (K < 2 || K > MTS_MAX_LEVELS) {
+        err = "a quadtree has 2 to " + std::to_string(MTS_MAX_LEVELS) + " levels";
+        return MTS_ERR_UNSUPPORTED;
+    }
     if (nx != ny || nx < 8 || (nx & (nx - 1)) != 0) {
         err = "Matrix dimensions must be divisible by 2 in each direction. (multi-block lattices are square, "
               "power-of-two sided)";
@@ -109,57 +116,68 @@ int multi_check(int nx, int ny, const int32_t* pore, std::string& err) {
     return MTS_OK;
 }
 
-// Level decision: an aligned 2x2 block is coarse when its four pixels are
-// fluid at a Euclidean distance >= 6 from every wall pixel.
-void multi_coarse_host(int nx, int ny, const int32_t* pore, bool torus, std::vector<uint8_t>& coarse) {
+// Split distance of the quadtree with at most K levels (grid.hpp:1180-1214):
+// a block of side h <= 2^(K-1) is a leaf when it is fluid and no wall pixel
+// lies closer than ceil(2^(K-1) sqrt 2) h to any of its pixels.
+int split_distance(int K) { return (int)std::ceil(std::pow(2.0, K - 1) * std::sqrt(2.0)); }
+
+// Level decision (the leaves of grid.hpp:1160-1269): each pixel's level is
+// 1 + log2 of the side of the largest aligned block holding it that is a
+// leaf; larger blocks are decided first, so a block inside a leaf is not
+// looked at again.
+void multi_levels_host(int nx, int ny, const int32_t* pore, bool torus, int K, std::vector<uint8_t>& lvl) {
     const std::vector<double> d2 = edt2(nx, ny, pore, torus);
     auto dist = [&](int x, int y) -> double {
         const double v = d2[(size_t)y * nx + x];
         return v >= 1e299 ? 1e30 : (double)(float)std::sqrt(v);
     };
-    const int cnx = nx / 2, cny = ny / 2;
-    coarse.assign((size_t)cnx * cny, 0);
-    for (int by = 0; by < cny; by++)
-        for (int bx = 0; bx < cnx; bx++) {
-            bool ok = true;
-            for (int dy = 0; dy < 2 && ok; dy++)
-                for (int dx = 0; dx < 2 && ok; dx++) {
-                    const int x = 2 * bx + dx, y = 2 * by + dy;
-                    ok = pore[(size_t)y * nx + x] == 1 && dist(x, y) >= 6.0;
-                }
-            coarse[(size_t)by * cnx + bx] = ok;
-        }
+    const int md = split_distance(K);
+    lvl.assign((size_t)nx * ny, 1);
+    for (int l = K; l >= 2; l--) {
+        const int h = 1 << (l - 1);
+        const double T = (double)md * h;
+        for (int by = 0; by + h <= ny; by += h)
+            for (int bx = 0; bx + h <= nx; bx += h) {
+                if (lvl[(size_t)by * nx + bx] != 1) continue;
+                bool ok = true;
+                for (int y = by; y < by + h && ok; y++)
+                    for (int x = bx; x < bx + h && ok; x++) ok = pore[(size_t)y * nx + x] == 1 && dist(x, y) >= T;
+                if (!ok) continue;
+                for (int y = by; y < by + h; y++)
+                    for (int x = bx; x < bx + h; x++) lvl[(size_t)y * nx + x] = (uint8_t)l;
+            }
+    }
 }
 
 int build_multi_grid(int nx, int ny, const int32_t* pore, const double* kn, const double* locpore, const Physics& ph,
-                     bool torus, MultiGeometry& G, std::string& err) {
+                     bool torus, int K, MultiGeometry& G, std::string& err) {
     (void)ph;
     (void)locpore;
     G = MultiGeometry();
-    if (int rc = multi_check(nx, ny, pore, err)) return rc;
-    std::vector<uint8_t> coarse;
-    multi_coarse_host(nx, ny, pore, torus, coarse);
-    if (int rc = multi_cells(nx, ny, pore, kn, torus, coarse, G, err)) return rc;
-    return multi_tables_host(G, coarse, err);
+    if (int rc = multi_check(nx, ny, pore, K, err)) return rc;
+    std::vector<uint8_t> lvl;
+    multi_levels_host(nx, ny, pore, torus, K, lvl);
+    if (int rc = multi_cells(nx, ny, pore, kn, torus, lvl, G, err)) return rc;
+    return multi_tables_host(G, err);
 }
 
-// The cells of the quadtree: original cells (fluid pixels outside coarse
-// blocks, coarse blocks) in row-major order of their top-left pixel, then
-// the buffer cells in creation order (grid.hpp:427-889; sequential: a
+// The cells of the quadtree: original cells (fluid pixels of level 1 and
+// the blocks of higher levels) in row-major order of their top-left pixel,
+// then the buffer cells in creation order (grid.hpp:427-889; sequential: a
 // buffer's placement depends on the buffers made before it).
-int multi_cells(int nx, int ny, const int32_t* pore, const double* kn, bool torus, const std::vector<uint8_t>& coarse,
+int multi_cells(int nx, int ny, const int32_t* pore, const double* kn, bool torus, const std::vector<uint8_t>& lvl,
                 MultiGeometry& G, std::string& err) {
     (void)err;
     G = MultiGeometry();
     G.nx = nx;
     G.ny = ny;
     G.torus = torus;
-    const int cnx = nx / 2, cny = ny / 2;
-    bool any = false;
-    for (uint8_t v : coarse) any = any || v;
-    G.maxLevel = any ? 2 : 1;
-    const int L = G.maxLevel;
-    auto gid = [&](int x, int y, int lvl) -> int64_t { return (int64_t)x + (int64_t)y * ny + (int64_t)(lvl - 1) * ny * nx; };
+    int L = 1;
+    for (uint8_t v : lvl) L = std::max<int>(L, v);
+    G.maxLevel = L;
+    G.lvl = lvl;
+    auto gid = [&](int x, int y, int l) -> int64_t { return (int64_t)x + (int64_t)y * ny + (int64_t)(l - 1) * ny * nx; };
+    auto lvlAt = [&](int x, int y) -> int { return lvl[(size_t)y * nx + x]; };
 
     // ---- original cells (incl. wall pixels) in row-major order of top-left pixel
     struct Orig {
@@ -170,113 +188,100 @@ int multi_cells(int nx, int ny, const int32_t* pore, const double* kn, bool toru
     std::vector<int32_t> origAt((size_t)nx * ny, -1);  // top-left pixel -> orig index
     for (int y = 0; y < ny; y++)
         for (int x = 0; x < nx; x++) {
-            const bool inCoarse = L > 1 && coarse[(size_t)(y / 2) * cnx + x / 2];
-            if (inCoarse) {
-                if ((x & 1) == 0 && (y & 1) == 0) {
-                    origAt[(size_t)y * nx + x] = (int32_t)orig.size();
-                    orig.push_back({x, y, 2, true});
-                }
-            } else {
-                origAt[(size_t)y * nx + x] = (int32_t)orig.size();
-                orig.push_back({x, y, 1, pore[(size_t)y * nx + x] == 1});
-            }
+            const int v = lvlAt(x, y), h = 1 << (v - 1);
+            if (v > 1 && (x % h || y % h)) continue;
+            origAt[(size_t)y * nx + x] = (int32_t)orig.size();
+            orig.push_back({x, y, v, v > 1 || pore[(size_t)y * nx + x] == 1});
         }
     // existence of a cell (original or buffer) per level position
-    std::vector<uint8_t> exist1((size_t)nx * ny, 0), exist2((size_t)cnx * cny, 0);
-    for (const Orig& o : orig) {
-        if (o.lvl == 1) exist1[(size_t)o.y * nx + o.x] = 1;
-        else exist2[(size_t)(o.y / 2) * cnx + o.x / 2] = 1;
-    }
-    // ---- buffer cells (grid.hpp:665-806), 2 levels
+    std::vector<std::vector<uint8_t>> exist(L);
+    for (int l = 0; l < L; l++) exist[l].assign((size_t)(nx >> l) * (ny >> l), 0);
+    auto ex = [&](int l, int x, int y) -> uint8_t& { return exist[l - 1][(size_t)(y >> (l - 1)) * (nx >> (l - 1)) + (x >> (l - 1))]; };
+    for (const Orig& o : orig) ex(o.lvl, o.x, o.y) = 1;
+    // ---- buffer cells (grid.hpp:598-806): a cell of level g >= 2 with a
+    // position of its level missing in its window (side 2 R, R the side of
+    // the largest cells) makes its children at level g - 1 buffers when a
+    // cell of its own level lies in range (it does itself), and for each
+    // other cell in range, when no larger cell does, a buffer of its level
+    // at the missing position nearest to that cell
     struct Buf {
         int x, y, lvl, type;
     };
     std::vector<Buf> bufs;
-    if (L > 1) {
-        for (const Orig& o : orig) {
-            if (o.lvl != 2) continue;
-            // 3x3 level-2 neighbourhood, column-major (x outer, y inner)
-            std::vector<std::pair<int, int>> nm;   // existing level-2 neighbours (top-left pixel)
-            std::vector<std::pair<int, int>> mk;   // missing positions (pixel coords, may be off-image on a torus: wrapped)
-            for (int ox = -2; ox <= 2; ox += 2)
-                for (int oy = -2; oy <= 2; oy += 2) {
-                    int x = o.x + ox, y = o.y + oy;
-                    if (!torus && (x < 0 || y < 0 || x >= nx || y >= ny)) continue;  // "-1" entries
-                    const int xw = wrapc(x, nx), yw = wrapc(y, ny);
-                    if (exist2[(size_t)(yw / 2) * cnx + xw / 2] &&
-                        // an original coarse cell (buffers are added later)
-                        origAt[(size_t)yw * nx + xw] >= 0 && orig[origAt[(size_t)yw * nx + xw]].lvl == 2)
-                        nm.push_back({x, y});
-                    else
-                        mk.push_back({x, y});
-                }
-            if (mk.empty()) continue;
-            // cells whose top-left pixel lies within +-2 of this cell's, in cell order
-            std::vector<std::pair<int, std::pair<int, int>>> inr;  // (orig index, relative position)
-            for (int oy = -2; oy <= 2; oy++)
-                for (int ox = -2; ox <= 2; ox++) {
-                    int x = o.x + ox, y = o.y + oy;
-                    if (!torus && (x < 0 || y < 0 || x >= nx || y >= ny)) continue;
-                    const int k = origAt[(size_t)wrapc(y, ny) * nx + wrapc(x, nx)];
-                    if (k >= 0) inr.push_back({k, {x, y}});
-                }
-            if (torus) {
-                // a cell may be reached twice through the wrap on tiny lattices; keep the first
-                std::sort(inr.begin(), inr.end(), [&](const auto& a, const auto& b) {
-                    return a.first != b.first ? a.first < b.first : a.second < b.second;
-                });
-                inr.erase(std::unique(inr.begin(), inr.end(), [](const auto& a, const auto& b) { return a.first == b.first; }),
-                          inr.end());
-            } else {
-                std::sort(inr.begin(), inr.end());
+    const int R = 1 << (L - 1);
+    for (const Orig& o : orig) {
+        const int g = o.lvl;
+        if (g < 2) continue;
+        const int jump = 1 << (g - 1);
+        // missing positions of the window, column-major (x outer, y inner),
+        // unwrapped (off-image positions of a torus wrap)
+        std::vector<std::pair<int, int>> mk;
+        for (int ox = -R; ox <= R; ox += jump)
+            for (int oy = -R; oy <= R; oy += jump) {
+                const int x = o.x + ox, y = o.y + oy;
+                if (!torus && (x < 0 || y < 0 || x >= nx || y >= ny)) continue;
+                if (lvlAt(wrapc(x, nx), wrapc(y, ny)) != g) mk.push_back({x, y});
             }
-            std::vector<std::pair<int, int>> missing = mk;
-            std::vector<bool> used(missing.size(), false);
-            for (const auto& e : inr) {
-                const Orig& c = orig[e.first];
-                const int rx = e.second.first, ry = e.second.second;
-                bool isNb = false;
-                if (c.lvl == 2)
-                    for (const auto& q : nm)
-                        if (wrapc(q.first, nx) == c.x && wrapc(q.second, ny) == c.y) isNb = true;
-                if (isNb) {
-                    // fine buffer children of this coarse cell (type 12)
-                    const int cand[4][2] = {{o.x, o.y}, {o.x + 1, o.y}, {o.x, o.y + 1}, {o.x + 1, o.y + 1}};
-                    for (auto& cd : cand)
-                        if (!exist1[(size_t)cd[1] * nx + cd[0]]) {
-                            exist1[(size_t)cd[1] * nx + cd[0]] = 1;
-                            bufs.push_back({cd[0], cd[1], 1, 12});
-                        }
-                } else {
-                    // coarse buffer at the missing position closest to this cell (type 21)
-                    int best = 0;
-                    double minD = std::numeric_limits<double>::max();
-                    for (size_t m = 0; m < missing.size(); m++) {
-                        float mx, my;
-                        if (used[m]) {
-                            // the reference marks a consumed key -1, whose position
-                            // ind2sub(-1) reads as (-1, 0); on a torus it is skipped
-                            if (torus) continue;
-                            mx = -1.0f;
-                            my = 0.0f;
-                        } else {
-                            mx = (float)missing[m].first;
-                            my = (float)missing[m].second;
-                        }
-                        const float ddx = ((float)rx + 0.5f) - (mx + 0.5f), ddy = ((float)ry + 0.5f) - (my + 0.5f);
-                        const double dd = std::sqrt((double)(std::pow(ddx, 2) + std::pow(ddy, 2)));
-                        if (dd < minD) {
-                            minD = dd;
-                            best = (int)m;
-                        }
+        if (mk.empty()) continue;
+        // cells whose top-left pixel lies within +-R of this cell's, in cell order
+        std::vector<std::pair<int, std::pair<int, int>>> inr;  // (orig index, unwrapped top-left)
+        for (int oy = -R; oy <= R; oy++)
+            for (int ox = -R; ox <= R; ox++) {
+                const int x = o.x + ox, y = o.y + oy;
+                if (!torus && (x < 0 || y < 0 || x >= nx || y >= ny)) continue;
+                const int k = origAt[(size_t)wrapc(y, ny) * nx + wrapc(x, nx)];
+                if (k >= 0) inr.push_back({k, {x, y}});
+            }
+        std::sort(inr.begin(), inr.end());
+        if (torus)  // a cell may be reached twice through the wrap on tiny lattices; keep the first
+            inr.erase(std::unique(inr.begin(), inr.end(), [](const auto& a, const auto& b) { return a.first == b.first; }),
+                      inr.end());
+        int maxL = 0;
+        for (const auto& e : inr) maxL = std::max(maxL, orig[e.first].lvl);
+        std::vector<bool> used(mk.size(), false);
+        for (const auto& e : inr) {
+            const Orig& c = orig[e.first];
+            if (c.lvl == g) {
+                // the children of this cell (type 10 (g - 1) + g)
+                const int hj = jump / 2;
+                const int cand[4][2] = {{o.x, o.y}, {o.x + hj, o.y}, {o.x, o.y + hj}, {o.x + hj, o.y + hj}};
+                for (auto& cd : cand)
+                    if (!ex(g - 1, cd[0], cd[1])) {
+                        ex(g - 1, cd[0], cd[1]) = 1;
+                        bufs.push_back({cd[0], cd[1], g - 1, 10 * (g - 1) + g});
                     }
-                    if (!used[best]) {
-                        const int mx = wrapc(missing[best].first, nx), my = wrapc(missing[best].second, ny);
-                        if (!exist2[(size_t)(my / 2) * cnx + mx / 2]) {
-                            exist2[(size_t)(my / 2) * cnx + mx / 2] = 1;
-                            bufs.push_back({mx, my, 2, 21});
-                            used[best] = true;
-                        }
+            } else if (maxL <= g) {
+                // a buffer of this level at the missing position closest to
+                // the cell in range (type 10 g + g - 1)
+                const int rx = e.second.first, ry = e.second.second;
+                const float shift = (float)(std::pow(2.0, g - 2) - 0.5);
+                int best = 0;
+                double minD = std::numeric_limits<double>::max();
+                for (size_t m = 0; m < mk.size(); m++) {
+                    float mx, my;
+                    if (used[m]) {
+                        // the reference marks a consumed key -1, whose position
+                        // ind2sub(-1) reads as (-1, 0); on a torus it is skipped
+                        if (torus) continue;
+                        mx = -1.0f;
+                        my = 0.0f;
+                    } else {
+                        mx = (float)mk[m].first;
+                        my = (float)mk[m].second;
+                    }
+                    const float ddx = ((float)rx + shift) - (mx + shift), ddy = ((float)ry + shift) - (my + shift);
+                    const double dd = std::sqrt((double)(std::pow(ddx, 2) + std::pow(ddy, 2)));
+                    if (dd < minD) {
+                        minD = dd;
+                        best = (int)m;
+                    }
+                }
+                if (!used[best]) {
+                    const int mx = wrapc(mk[best].first, nx), my = wrapc(mk[best].second, ny);
+                    if (!ex(g, mx, my)) {
+                        ex(g, mx, my) = 1;
+                        bufs.push_back({mx, my, g, 10 * g + g - 1});
+                        used[best] = true;
                     }
                 }
             }
@@ -313,38 +318,25 @@ int multi_cells(int nx, int ny, const int32_t* pore, const double* kn, bool toru
 
 // Neighbour and interface tables of the cells (grid.hpp:916-1157):
 // connectivity, boundary types, parent/child rows and the coarse and fine
-// buffer rows.  The device builder (mts_quadtree.hip) forms the same tables.
-int multi_tables_host(MultiGeometry& G, const std::vector<uint8_t>& coarse, std::string& err) {
-    const int nx = G.nx, ny = G.ny, cnx = nx / 2, cny = ny / 2, L = G.maxLevel;
+// buffer rows of each pair of levels.  The device builder (mts_quadtree.hip)
+// forms the same tables.
+int multi_tables_host(MultiGeometry& G, std::string& err) {
+    const int nx = G.nx, ny = G.ny, L = G.maxLevel;
     const int64_t n = G.ncell;
-    // per-level index images: cell at a level position
-    std::vector<int64_t> at1((size_t)nx * ny, -1), at2((size_t)cnx * cny, -1);
-    std::vector<uint8_t> orig1((size_t)nx * ny, 0), orig2((size_t)cnx * cny, 0);
-    for (int64_t c = n - 1; c >= 0; c--) {
-        if (G.level[c] == 1) at1[(size_t)G.cy[c] * nx + G.cx[c]] = c;
-        else at2[(size_t)(G.cy[c] / 2) * cnx + G.cx[c] / 2] = c;
-    }
-    // positions of original cells (incl. wall pixels) per level: a pixel
-    // outside the coarse blocks, a coarse block
-    for (int y = 0; y < ny; y++)
-        for (int x = 0; x < nx; x++) {
-            const bool inCoarse = L > 1 && coarse[(size_t)(y / 2) * cnx + x / 2];
-            if (inCoarse) orig2[(size_t)(y / 2) * cnx + x / 2] = 1;
-            else orig1[(size_t)y * nx + x] = 1;
-        }
-    // ---- connectivity (grid.hpp:916-972)
+    // per-level index images: the lowest cell at a level position
+    std::vector<std::vector<int64_t>> at(L);
+    for (int l = 0; l < L; l++) at[l].assign((size_t)(nx >> l) * (ny >> l), -1);
+    auto atp = [&](int l, int x, int y) -> int64_t& { return at[l - 1][(size_t)(y >> (l - 1)) * (nx >> (l - 1)) + (x >> (l - 1))]; };
+    for (int64_t c = n - 1; c >= 0; c--) atp(G.level[c], G.cx[c], G.cy[c]) = c;
+    // ---- connectivity (grid.hpp:916-972): -1 where an original cell (a
+    // wall pixel) of the level sits, -2 where none does
     G.connect.assign(n * 9, -2);
     for (int64_t c = 0; c < n; c++) {
-        const int jump = G.level[c] == 1 ? 1 : 2;
+        const int l = G.level[c], jump = 1 << (l - 1);
         for (int k = 0; k < 9; k++) {
             const int x = wrapc(G.cx[c] + EX[k] * jump, nx), y = wrapc(G.cy[c] + EY[k] * jump, ny);
-            if (G.level[c] == 1) {
-                const int64_t q = at1[(size_t)y * nx + x];
-                G.connect[c * 9 + k] = q >= 0 ? q : (orig1[(size_t)y * nx + x] ? -1 : -2);
-            } else {
-                const int64_t q = at2[(size_t)(y / 2) * cnx + x / 2];
-                G.connect[c * 9 + k] = q >= 0 ? q : (orig2[(size_t)(y / 2) * cnx + x / 2] ? -1 : -2);
-            }
+            const int64_t q = atp(l, x, y);
+            G.connect[c * 9 + k] = q >= 0 ? q : (G.lvl[(size_t)y * nx + x] == l ? -1 : -2);
         }
     }
     // ---- boundary types of non-buffer cells (grid.hpp:1111-1157)
@@ -366,29 +358,42 @@ int multi_tables_host(MultiGeometry& G, const std::vector<uint8_t>& coarse, std:
         }
         G.bnd[c] = t;
     }
-    // ---- parent/child (grid.hpp:974-1038) and buffer rows (grid.hpp:1041-1108)
+    // ---- parent/child (grid.hpp:974-1038): the four cells of the level
+    // below in the cell's quadrants
     std::vector<int64_t> rowOf(n, -1);
+    G.parentChild.clear();
     for (int64_t c = 0; c < n; c++) {
         if (G.level[c] <= 1) continue;
+        const int hj = 1 << (G.level[c] - 2);
         rowOf[c] = (int64_t)G.parentChild.size() / 5;
         G.parentChild.push_back(c);
         const int sub[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
-        for (auto& s : sub) G.parentChild.push_back(at1[(size_t)(G.cy[c] + s[1]) * nx + G.cx[c] + s[0]]);
+        for (auto& s : sub) G.parentChild.push_back(atp(G.level[c] - 1, G.cx[c] + s[0] * hj, G.cy[c] + s[1] * hj));
     }
-    for (int64_t c = 0; c < n; c++)
-        if (G.type[c] == 21) {
-            const int64_t r = rowOf[c];
-            G.coarseRows.insert(G.coarseRows.end(), G.parentChild.begin() + 5 * r, G.parentChild.begin() + 5 * r + 5);
-        }
-    {
+    // ---- buffer rows (grid.hpp:1041-1108) per pair of levels (l, l + 1):
+    // coarse buffers (type 10 (l + 1) + l) with their children, fine
+    // buffers (type 10 l + l + 1) four at a time under the parent of the
+    // first row naming any of them
+    G.coarseRows.clear();
+    G.fineRows.clear();
+    G.coarseCount.assign(std::max(L - 1, 0), 0);
+    G.fineCount.assign(std::max(L - 1, 0), 0);
+    std::unordered_map<int64_t, int64_t> firstRow;
+    const int64_t rows = (int64_t)G.parentChild.size() / 5;
+    for (int64_t r = rows - 1; r >= 0; r--)
+        for (int k = 4; k >= 1; k--)
+            if (G.parentChild[5 * r + k] >= 0) firstRow[G.parentChild[5 * r + k]] = r;
+    for (int l = 1; l < L; l++) {
+        const int ctype = 10 * (l + 1) + l, ftype = 10 * l + l + 1;
+        for (int64_t c = 0; c < n; c++)
+            if (G.type[c] == ctype) {
+                const int64_t r = rowOf[c];
+                G.coarseRows.insert(G.coarseRows.end(), G.parentChild.begin() + 5 * r, G.parentChild.begin() + 5 * r + 5);
+                G.coarseCount[l - 1]++;
+            }
         std::vector<int64_t> ids;
         for (int64_t c = 0; c < n; c++)
-            if (G.type[c] == 12) ids.push_back(c);
-        std::unordered_map<int64_t, int64_t> firstRow;
-        const int64_t rows = (int64_t)G.parentChild.size() / 5;
-        for (int64_t r = rows - 1; r >= 0; r--)
-            for (int k = 4; k >= 1; k--)
-                if (G.parentChild[5 * r + k] >= 0) firstRow[G.parentChild[5 * r + k]] = r;
+            if (G.type[c] == ftype) ids.push_back(c);
         for (size_t g4 = 0; g4 + 3 < ids.size(); g4 += 4) {
             int64_t best = std::numeric_limits<int64_t>::max();
             for (int k = 0; k < 4; k++) {
@@ -401,6 +406,7 @@ int multi_tables_host(MultiGeometry& G, const std::vector<uint8_t>& coarse, std:
             }
             const int64_t row[5] = {G.parentChild[5 * best], ids[g4], ids[g4 + 1], ids[g4 + 2], ids[g4 + 3]};
             G.fineRows.insert(G.fineRows.end(), row, row + 5);
+            G.fineCount[l - 1]++;
         }
     }
     return MTS_OK;

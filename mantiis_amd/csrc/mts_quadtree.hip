@@ -31,43 +31,52 @@ __device__ __forceinline__ int wrapd(int v, int n) { return ((v % n) + n) % n; }
 __device__ __forceinline__ int exq(int k) { return (k == 1 || k == 5 || k == 8) ? 1 : ((k == 3 || k == 6 || k == 7) ? -1 : 0); }
 __device__ __forceinline__ int eyq(int k) { return (k == 2 || k == 5 || k == 6) ? 1 : ((k == 4 || k == 7 || k == 8) ? -1 : 0); }
 
-// Level decision (grid.hpp:1180-1214 at two levels): an aligned 2 x 2 block
-// is coarse when its pixels are fluid and no wall pixel lies at a squared
-// distance below 36 from any of them (the exact-distance test d >= 6 of the
-// host's Euclidean transform).  Wall pixels beyond the image edge do not
-// exist unless the lattice is a torus.
-__global__ void k_coarse(int nx, int ny, int torus, const int32_t* __restrict__ pore, uint8_t* __restrict__ coarse) {
-    const int cnx = nx / 2, cny = ny / 2;
+// Level decision (grid.hpp:1160-1269): launched for l = K .. 2 in turn; an
+// aligned block of side h = 2^(l-1) not inside a larger leaf becomes a leaf
+// of level l when its pixels are fluid and no wall pixel lies at a squared
+// distance below T^2 (T = ceil(2^(K-1) sqrt 2) h) from any of them -- the
+// exact-distance test d >= T of the host's Euclidean transform.  Wall
+// pixels beyond the image edge do not exist unless the lattice is a torus.
+__global__ void k_levels(int nx, int ny, int torus, int l, int T, const int32_t* __restrict__ pore,
+                         uint8_t* __restrict__ lvl) {
+    const int h = 1 << (l - 1), bnx = nx / h, bny = ny / h;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= cnx * cny) return;
-    const int bx = b % cnx, by = b / cnx;
+    if (b >= bnx * bny) return;
+    const int x0 = (b % bnx) * h, y0 = (b / bnx) * h;
+    if (lvl[(size_t)y0 * nx + x0] != 1) return;  // inside a larger leaf
     bool ok = true;
-    for (int q = 0; q < 4 && ok; q++) {
-        const int x = 2 * bx + (q & 1), y = 2 * by + (q >> 1);
-        ok = pore[(size_t)y * nx + x] == 1;
-        for (int dy = -5; dy <= 5 && ok; dy++)
-            for (int dx = -5; dx <= 5 && ok; dx++) {
-                if (dx * dx + dy * dy >= 36) continue;
-                int xx = x + dx, yy = y + dy;
-                if (torus) {
-                    xx = wrapd(xx, nx);
-                    yy = wrapd(yy, ny);
-                } else if (xx < 0 || yy < 0 || xx >= nx || yy >= ny) {
-                    continue;
-                }
-                if (pore[(size_t)yy * nx + xx] == 0) ok = false;
-            }
+    const int T2 = T * T;
+    for (int oy = -(T - 1); oy <= h - 1 + (T - 1) && ok; oy++) {
+        const int ddy = oy < 0 ? -oy : (oy >= h ? oy - h + 1 : 0);
+        int yy = y0 + oy;
+        if (torus) yy = wrapd(yy, ny);
+        else if (yy < 0 || yy >= ny) continue;
+        for (int ox = -(T - 1); ox <= h - 1 + (T - 1) && ok; ox++) {
+            const int ddx = ox < 0 ? -ox : (ox >= h ? ox - h + 1 : 0);
+            if (ddx * ddx + ddy * ddy >= T2) continue;
+            int xx = x0 + ox;
+            if (torus) xx = wrapd(xx, nx);
+            else if (xx < 0 || xx >= nx) continue;
+            if (pore[(size_t)yy * nx + xx] != 1) ok = false;
+        }
     }
-    coarse[b] = ok ? 1 : 0;
+    if (!ok) return;
+    for (int y = y0; y < y0 + h; y++)
+        for (int x = x0; x < x0 + h; x++) lvl[(size_t)y * nx + x] = (uint8_t)l;
 }
 
 struct Cells {
-    int nx, ny, cnx, L;
+    int nx, ny, L;
     int64_t n;
     const int32_t *cx, *cy, *level, *type, *isBuffer;
-    const uint8_t* coarse;
-    int32_t *at1, *at2;  // per level position: the lowest cell there (INT_MAX: none)
+    const uint8_t* lvl;             // level of each pixel's leaf
+    int32_t* at[MTS_MAX_LEVELS];    // per level position: the lowest cell there (INT_MAX: none)
+    int lnx[MTS_MAX_LEVELS];        // level image widths
 };
+
+__device__ __forceinline__ int32_t& at_of(const Cells& C, int l, int x, int y) {
+    return C.at[l - 1][(size_t)(y >> (l - 1)) * C.lnx[l - 1] + (x >> (l - 1))];
+}
 
 __global__ void k_fill(int32_t* p, int64_t n, int32_t v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,29 +86,21 @@ __global__ void k_fill(int32_t* p, int64_t n, int32_t v) {
 __global__ void k_index(Cells C) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.n) return;
-    if (C.level[c] == 1) atomicMin(&C.at1[(size_t)C.cy[c] * C.nx + C.cx[c]], (int32_t)c);
-    else atomicMin(&C.at2[(size_t)(C.cy[c] / 2) * C.cnx + C.cx[c] / 2], (int32_t)c);
+    atomicMin(&at_of(C, C.level[c], C.cx[c], C.cy[c]), (int32_t)c);
 }
 
 // grid.hpp:916-972: neighbour k of cell c at its level's spacing (wrapped),
-// -1 when an original cell (a wall pixel) sits there, -2 when nothing does.
+// -1 when an original cell of that level (a wall pixel) sits there, -2 when
+// nothing does.
 __global__ void k_connect(Cells C, int64_t* __restrict__ connect) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C.n * 9) return;
     const int64_t c = i / 9;
     const int k = (int)(i - c * 9);
-    const int jump = C.level[c] == 1 ? 1 : 2;
+    const int l = C.level[c], jump = 1 << (l - 1);
     const int x = wrapd(C.cx[c] + exq(k) * jump, C.nx), y = wrapd(C.cy[c] + eyq(k) * jump, C.ny);
-    const bool inCoarse = C.L > 1 && C.coarse[(size_t)(y / 2) * C.cnx + x / 2];
-    int64_t v;
-    if (C.level[c] == 1) {
-        const int32_t q = C.at1[(size_t)y * C.nx + x];
-        v = q != INT_MAX ? (int64_t)q : (!inCoarse ? -1 : -2);
-    } else {
-        const int32_t q = C.at2[(size_t)(y / 2) * C.cnx + x / 2];
-        v = q != INT_MAX ? (int64_t)q : (inCoarse ? -1 : -2);
-    }
-    connect[i] = v;
+    const int32_t q = at_of(C, l, x, y);
+    connect[i] = q != INT_MAX ? (int64_t)q : (C.lvl[(size_t)y * C.nx + x] == l ? -1 : -2);
 }
 
 // grid.hpp:1111-1157: the boundary type of a non-buffer cell from the
@@ -123,25 +124,26 @@ __global__ void k_bnd(Cells C, const int64_t* __restrict__ connect, int32_t* __r
     if (t == -2) atomicMin(bad, (unsigned long long)c);
 }
 
-// grid.hpp:974-1038: row rowOf[c] of a level-2 cell {c, children tl, tr,
-// bl, br} (-1 where no fine cell is).
+// grid.hpp:974-1038: row rowOf[c] of a cell of level >= 2 {c, children
+// tl, tr, bl, br} (-1 where no cell of the level below is).
 __global__ void k_parent(Cells C, const int32_t* __restrict__ rowOf, int64_t* __restrict__ pc) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.n || C.level[c] <= 1) return;
     const int64_t r = rowOf[c];
+    const int l = C.level[c], hj = 1 << (l - 2);
     pc[5 * r] = c;
     for (int s = 0; s < 4; s++) {
-        const int32_t q = C.at1[(size_t)(C.cy[c] + (s >> 1)) * C.nx + C.cx[c] + (s & 1)];
+        const int32_t q = at_of(C, l - 1, C.cx[c] + (s & 1) * hj, C.cy[c] + (s >> 1) * hj);
         pc[5 * r + 1 + s] = q != INT_MAX ? (int64_t)q : -1;
     }
 }
 
-// Coarse buffer rows (type 21): the parent/child row of each, in cell order.
-__global__ void k_coarse_rows(Cells C, const int32_t* __restrict__ rowOf, const int32_t* __restrict__ pos21,
+// Coarse buffer rows of one type: the parent/child row of each, in cell order.
+__global__ void k_coarse_rows(Cells C, int ctype, const int32_t* __restrict__ rowOf, const int32_t* __restrict__ pos,
                               const int64_t* __restrict__ pc, int64_t* __restrict__ rows) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C.n || C.type[c] != 21) return;
-    const int64_t r = rowOf[c], o = pos21[c];
+    if (c >= C.n || C.type[c] != ctype) return;
+    const int64_t r = rowOf[c], o = pos[c];
     for (int k = 0; k < 5; k++) rows[5 * o + k] = pc[5 * r + k];
 }
 
@@ -155,10 +157,10 @@ __global__ void k_first_row(int64_t nrows, const int64_t* __restrict__ pc, int32
     }
 }
 
-// grid.hpp:1041-1108: the fine buffer cells (type 12) in cell order ...
-__global__ void k_fine_ids(Cells C, const int32_t* __restrict__ pos12, int64_t* __restrict__ ids) {
+// grid.hpp:1041-1108: the fine buffer cells of one type in cell order ...
+__global__ void k_fine_ids(Cells C, int ftype, const int32_t* __restrict__ pos, int64_t* __restrict__ ids) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < C.n && C.type[c] == 12) ids[pos12[c]] = c;
+    if (c < C.n && C.type[c] == ftype) ids[pos[c]] = c;
 }
 
 // ... four at a time, under the parent of the first row naming any of the four.
@@ -181,8 +183,8 @@ __global__ void k_fine_groups(const int32_t* __restrict__ first, const int64_t* 
 // cell ORs its own word into its site and its neighbours' "in" bits into
 // theirs.
 struct Meta {
-    uint32_t* meta[2];
-    int lnx[2];
+    uint32_t* meta[MTS_MAX_LEVELS];
+    int lnx[MTS_MAX_LEVELS];
 };
 __global__ void k_meta(Cells C, const int64_t* __restrict__ connect, const int32_t* __restrict__ bnd, int torus,
                        int half, Meta M) {
@@ -268,53 +270,60 @@ __global__ void k_flag(const int32_t* __restrict__ a, int64_t n, int mode, int32
 
 }  // namespace
 
-int multi_coarse_device(int nx, int ny, const int32_t* pore, bool torus, hipStream_t st, std::vector<uint8_t>& coarse,
-                        std::string& err) {
+int multi_levels_device(int nx, int ny, const int32_t* pore, bool torus, int K, hipStream_t st,
+                        std::vector<uint8_t>& lvl, std::string& err) {
     DevBuf B;
     hipError_t e = hipSuccess;
-    const int64_t P = (int64_t)nx * ny, nb = (int64_t)(nx / 2) * (ny / 2);
+    const int64_t P = (int64_t)nx * ny;
     int32_t* d_pore = B.get<int32_t>(P, e);
-    uint8_t* d_coarse = B.get<uint8_t>(nb, e);
+    uint8_t* d_lvl = B.get<uint8_t>(P, e);
     if (e == hipSuccess) e = hipMemcpyAsync(d_pore, pore, P * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-        qt::k_coarse<<<grid_of(nb), 256, 0, st>>>(nx, ny, torus ? 1 : 0, d_pore, d_coarse);
+    if (e == hipSuccess) e = hipMemsetAsync(d_lvl, 1, P, st);
+    const int md = split_distance(K);
+    for (int l = K; l >= 2 && e == hipSuccess; l--) {
+        const int h = 1 << (l - 1);
+        const int64_t nb = (int64_t)(nx / h) * (ny / h);
+        qt::k_levels<<<grid_of(nb), 256, 0, st>>>(nx, ny, torus ? 1 : 0, l, md * h, d_pore, d_lvl);
         e = hipGetLastError();
     }
-    coarse.assign(nb, 0);
-    if (e == hipSuccess) e = hipMemcpyAsync(coarse.data(), d_coarse, nb, hipMemcpyDeviceToHost, st);
+    lvl.assign(P, 1);
+    if (e == hipSuccess) e = hipMemcpyAsync(lvl.data(), d_lvl, P, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e == hipSuccess ? MTS_OK : hip_fail(e, "quadtree level decision", err);
 }
 
-int multi_tables_device(MultiGeometry& G, const std::vector<uint8_t>& coarse, std::vector<LevelImage>* levels,
-                        hipStream_t st, std::string& err) {
-    const int nx = G.nx, ny = G.ny, cnx = nx / 2, cny = ny / 2;
+int multi_tables_device(MultiGeometry& G, std::vector<LevelImage>* levels, hipStream_t st, std::string& err) {
+    const int nx = G.nx, ny = G.ny, L = G.maxLevel;
     const int64_t n = G.ncell;
     if (n == 0) {  // no fluid cell: no tables (as the host builder leaves them)
-        return multi_tables_host(G, coarse, err);
+        return multi_tables_host(G, err);
+    }
+    if (L > MTS_MAX_LEVELS) {
+        err = "quadtree tables: too many levels";
+        return MTS_ERR_UNSUPPORTED;
     }
     DevBuf B;
     hipError_t e = hipSuccess;
     qt::Cells C;
     C.nx = nx;
     C.ny = ny;
-    C.cnx = cnx;
-    C.L = G.maxLevel;
+    C.L = L;
     C.n = n;
     int32_t* cx = B.get<int32_t>(n, e);
     int32_t* cy = B.get<int32_t>(n, e);
     int32_t* level = B.get<int32_t>(n, e);
     int32_t* type = B.get<int32_t>(n, e);
     int32_t* isBuf = B.get<int32_t>(n, e);
-    uint8_t* d_coarse = B.get<uint8_t>(coarse.size(), e);
-    int32_t* at1 = B.get<int32_t>((size_t)nx * ny, e);
-    int32_t* at2 = B.get<int32_t>((size_t)cnx * cny, e);
+    uint8_t* d_lvl = B.get<uint8_t>((size_t)nx * ny, e);
+    for (int l = 0; l < MTS_MAX_LEVELS; l++) {
+        C.lnx[l] = nx >> l;
+        C.at[l] = l < L ? B.get<int32_t>((size_t)(nx >> l) * (ny >> l), e) : nullptr;
+    }
     int64_t* connect = B.get<int64_t>((size_t)n * 9, e);
     int32_t* bnd = B.get<int32_t>(n, e);
     int32_t* flag = B.get<int32_t>(n, e);
     int32_t* rowOf = B.get<int32_t>(n, e);
-    int32_t* pos21 = B.get<int32_t>(n, e);
-    int32_t* pos12 = B.get<int32_t>(n, e);
+    int32_t* pos = B.get<int32_t>(n, e);
     int32_t* first = B.get<int32_t>(n, e);
     unsigned long long* bad = B.get<unsigned long long>(2, e);
     if (e != hipSuccess) return hip_fail(e, "quadtree tables: device memory", err);
@@ -326,8 +335,7 @@ int multi_tables_device(MultiGeometry& G, const std::vector<uint8_t>& coarse, st
     up(level, G.level);
     up(type, G.type);
     up(isBuf, G.isBuffer);
-    if (e == hipSuccess && !coarse.empty())
-        e = hipMemcpyAsync(d_coarse, coarse.data(), coarse.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_lvl, G.lvl.data(), G.lvl.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(qt::c_bndcode), MTS_BND_CODE, sizeof(MTS_BND_CODE), 0,
                                                     hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(qt::c_ifstream), MTS_IFSTREAM, sizeof(MTS_IFSTREAM), 0,
@@ -340,58 +348,76 @@ int multi_tables_device(MultiGeometry& G, const std::vector<uint8_t>& coarse, st
     C.level = level;
     C.type = type;
     C.isBuffer = isBuf;
-    C.coarse = d_coarse;
-    C.at1 = at1;
-    C.at2 = at2;
-    qt::k_fill<<<grid_of((int64_t)nx * ny), 256, 0, st>>>(at1, (int64_t)nx * ny, INT_MAX);
-    qt::k_fill<<<grid_of((int64_t)cnx * cny), 256, 0, st>>>(at2, (int64_t)cnx * cny, INT_MAX);
+    C.lvl = d_lvl;
+    for (int l = 0; l < L; l++) {
+        const int64_t P = (int64_t)(nx >> l) * (ny >> l);
+        qt::k_fill<<<grid_of(P), 256, 0, st>>>(C.at[l], P, INT_MAX);
+    }
     qt::k_fill<<<grid_of(n), 256, 0, st>>>(first, n, INT_MAX);
     qt::k_index<<<grid_of(n), 256, 0, st>>>(C);
     qt::k_connect<<<grid_of(n * 9), 256, 0, st>>>(C, connect);
     qt::k_bnd<<<grid_of(n), 256, 0, st>>>(C, connect, bnd, bad);
-    // parent/child rows: one per level-2 cell, in cell order
+    // parent/child rows: one per cell of level >= 2, in cell order
     k_flag<<<grid_of(n), 256, 0, st>>>(level, n, 0, flag);
     e = hipGetLastError();
     if (e == hipSuccess) e = exclusive_sum(flag, rowOf, n, st, B);
-    k_flag<<<grid_of(n), 256, 0, st>>>(type, n, 21, flag);
-    if (e == hipSuccess) e = exclusive_sum(flag, pos21, n, st, B);
-    k_flag<<<grid_of(n), 256, 0, st>>>(type, n, 12, flag);
-    if (e == hipSuccess) e = exclusive_sum(flag, pos12, n, st, B);
     if (e != hipSuccess) return hip_fail(e, "quadtree tables: scans", err);
-    int64_t nrows = 0, n21 = 0, n12 = 0;
+    int64_t nrows = 0;
+    std::vector<int64_t> nct(std::max(L - 1, 0), 0), nft(std::max(L - 1, 0), 0);
     for (int64_t c = 0; c < n; c++) {
         nrows += G.level[c] > 1;
-        n21 += G.type[c] == 21;
-        n12 += G.type[c] == 12;
+        for (int l = 1; l < L; l++) {
+            nct[l - 1] += G.type[c] == 10 * (l + 1) + l;
+            nft[l - 1] += G.type[c] == 10 * l + l + 1;
+        }
     }
-    const int64_t ngroups = n12 / 4;
+    int64_t ncrow = 0, nfrow = 0, nfid = 0;
+    for (int l = 1; l < L; l++) {
+        ncrow += nct[l - 1];
+        nfrow += nft[l - 1] / 4;
+        nfid = std::max(nfid, nft[l - 1]);
+    }
     int64_t* pc = B.get<int64_t>((size_t)nrows * 5, e);
-    int64_t* crow = B.get<int64_t>((size_t)n21 * 5, e);
-    int64_t* ids = B.get<int64_t>((size_t)n12, e);
-    int64_t* frow = B.get<int64_t>((size_t)ngroups * 5, e);
+    int64_t* crow = B.get<int64_t>((size_t)ncrow * 5, e);
+    int64_t* ids = B.get<int64_t>((size_t)nfid, e);
+    int64_t* frow = B.get<int64_t>((size_t)nfrow * 5, e);
     if (e != hipSuccess) return hip_fail(e, "quadtree tables: device memory", err);
     qt::k_parent<<<grid_of(n), 256, 0, st>>>(C, rowOf, pc);
-    qt::k_coarse_rows<<<grid_of(n), 256, 0, st>>>(C, rowOf, pos21, pc, crow);
     qt::k_first_row<<<grid_of(nrows), 256, 0, st>>>(nrows, pc, first);
-    qt::k_fine_ids<<<grid_of(n), 256, 0, st>>>(C, pos12, ids);
-    qt::k_fine_groups<<<grid_of(ngroups), 256, 0, st>>>(first, pc, ngroups, ids, frow, bad + 1);
-    e = hipGetLastError();
+    // interface rows per pair of levels, one pair after the other
+    int64_t co = 0, fo = 0;
+    for (int l = 1; l < L && e == hipSuccess; l++) {
+        const int ctype = 10 * (l + 1) + l, ftype = 10 * l + l + 1;
+        k_flag<<<grid_of(n), 256, 0, st>>>(type, n, ctype, flag);
+        e = exclusive_sum(flag, pos, n, st, B);
+        if (e != hipSuccess) break;
+        qt::k_coarse_rows<<<grid_of(n), 256, 0, st>>>(C, ctype, rowOf, pos, pc, crow + 5 * co);
+        k_flag<<<grid_of(n), 256, 0, st>>>(type, n, ftype, flag);
+        e = exclusive_sum(flag, pos, n, st, B);
+        if (e != hipSuccess) break;
+        const int64_t ngroups = nft[l - 1] / 4;
+        qt::k_fine_ids<<<grid_of(n), 256, 0, st>>>(C, ftype, pos, ids);
+        qt::k_fine_groups<<<grid_of(ngroups), 256, 0, st>>>(first, pc, ngroups, ids, frow + 5 * fo, bad + 1);
+        e = hipGetLastError();
+        co += nct[l - 1];
+        fo += ngroups;
+    }
     // link words of the level images
-    uint32_t* meta[2] = {nullptr, nullptr};
-    if (levels) {
+    uint32_t* meta[MTS_MAX_LEVELS] = {};
+    if (levels && e == hipSuccess) {
         qt::Meta M;
-        for (int l = 0; l < 2; l++) {
+        for (int l = 0; l < MTS_MAX_LEVELS; l++) {
             M.meta[l] = nullptr;
             M.lnx[l] = nx >> l;
         }
-        for (int l = 0; l < G.maxLevel && l < 2; l++) {
+        for (int l = 0; l < L; l++) {
             const int64_t P = (int64_t)(nx >> l) * (ny >> l);
             meta[l] = B.get<uint32_t>(P, e);
             if (e == hipSuccess) qt::k_fill<<<grid_of(P), 256, 0, st>>>((int32_t*)meta[l], P, 0);
             M.meta[l] = meta[l];
         }
         if (e == hipSuccess) {
-            qt::k_meta<<<grid_of(n), 256, 0, st>>>(C, connect, bnd, G.torus ? 1 : 0, 1 << (G.maxLevel - 1), M);
+            qt::k_meta<<<grid_of(n), 256, 0, st>>>(C, connect, bnd, G.torus ? 1 : 0, 1 << (L - 1), M);
             e = hipGetLastError();
         }
     }
@@ -401,8 +427,11 @@ int multi_tables_device(MultiGeometry& G, const std::vector<uint8_t>& coarse, st
     G.connect.resize((size_t)n * 9);
     G.bnd.resize(n);
     G.parentChild.resize((size_t)nrows * 5);
-    G.coarseRows.resize((size_t)n21 * 5);
-    G.fineRows.resize((size_t)ngroups * 5);
+    G.coarseRows.resize((size_t)ncrow * 5);
+    G.fineRows.resize((size_t)nfrow * 5);
+    G.coarseCount = nct;
+    G.fineCount.assign(std::max(L - 1, 0), 0);
+    for (int l = 1; l < L; l++) G.fineCount[l - 1] = nft[l - 1] / 4;
     auto down = [&](void* h, const void* d, size_t bytes) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st);
     };
@@ -413,7 +442,7 @@ int multi_tables_device(MultiGeometry& G, const std::vector<uint8_t>& coarse, st
     down(G.coarseRows.data(), crow, G.coarseRows.size() * 8);
     down(G.fineRows.data(), frow, G.fineRows.size() * 8);
     if (levels)
-        for (int l = 0; l < G.maxLevel && l < 2; l++) {
+        for (int l = 0; l < L; l++) {
             LevelImage& I = (*levels)[l];
             down(I.meta.data(), meta[l], I.meta.size() * 4);
         }
@@ -444,8 +473,9 @@ int mts_quadtree_check(int nx, int ny, const int32_t* pore, const double* kn, co
     if (p) prm = *p;
     else mts_default_params(&prm);
     const bool torus = prm.periodic_y != 0;
+    const int K = quadtree_levels(prm.multiblock);
     std::string err;
-    int rc = multi_check(nx, ny, pore, err);
+    int rc = multi_check(nx, ny, pore, K, err);
     if (rc) return set_error(rc, err);
     hipError_t e = hipSetDevice(prm.device);
     if (e != hipSuccess) return set_error(MTS_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
@@ -454,12 +484,12 @@ int mts_quadtree_check(int nx, int ny, const int32_t* pore, const double* kn, co
     if (e != hipSuccess) return set_error(MTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
     // host: transform-based level decision and the host tables
     std::vector<uint8_t> ch, cd;
-    multi_coarse_host(nx, ny, pore, torus, ch);
+    multi_levels_host(nx, ny, pore, torus, K, ch);
     MultiGeometry H, D;
     rc = multi_cells(nx, ny, pore, kn, torus, ch, H, err);
-    if (!rc) rc = multi_tables_host(H, ch, err);
+    if (!rc) rc = multi_tables_host(H, err);
     // device: window-based level decision and the device tables
-    if (!rc) rc = multi_coarse_device(nx, ny, pore, torus, st, cd, err);
+    if (!rc) rc = multi_levels_device(nx, ny, pore, torus, K, st, cd, err);
     if (!rc) rc = multi_cells(nx, ny, pore, kn, torus, cd, D, err);
     std::vector<LevelImage> LH, LD;
     std::vector<CellParam> par;
@@ -471,7 +501,7 @@ int mts_quadtree_check(int nx, int ny, const int32_t* pore, const double* kn, co
         rc = build_level_images(H, LH, par, ph, err, true);
     }
     if (!rc) rc = build_level_images(D, LD, par, ph, err, false);
-    if (!rc) rc = multi_tables_device(D, cd, &LD, st, err);
+    if (!rc) rc = multi_tables_device(D, &LD, st, err);
     (void)hipStreamDestroy(st);
     if (rc) return set_error(rc, err);
     int64_t d = 0;
@@ -492,6 +522,8 @@ int mts_quadtree_check(int nx, int ny, const int32_t* pore, const double* kn, co
     cmp(H.parentChild, D.parentChild);
     cmp(H.coarseRows, D.coarseRows);
     cmp(H.fineRows, D.fineRows);
+    cmp(H.coarseCount, D.coarseCount);
+    cmp(H.fineCount, D.fineCount);
     if (LH.size() != LD.size()) d += 1;
     for (size_t l = 0; l < LH.size() && l < LD.size(); l++) cmp(LH[l].meta, LD[l].meta);
     *ndiff = d;

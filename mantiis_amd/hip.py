@@ -345,7 +345,9 @@ class HostGrid:
         return out
 
     def rows(self, which):
-        w = {"parent_child": 0, "coarse": 1, "fine": 2}[which]
+        """which: "parent_child", "coarse" (every pair's coarse buffer rows),
+        "fine", or a buffer type (21, 32: coarse; 12, 23: fine)."""
+        w = which if isinstance(which, int) else {"parent_child": 0, "coarse": 1, "fine": 2}[which]
         cnt = C.c_int64()
         _check(self.lib, self.lib.mts_grid_rows(self.h, w, None, C.byref(cnt)))
         out = np.zeros((cnt.value, 5), np.int64)

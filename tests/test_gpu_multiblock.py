@@ -183,6 +183,25 @@ def test_device_quadtree_tables_equal_host(name, geom, torus):
     assert quadtree_check(pore, kn, periodic_y=int(torus)) == 0
 
 
+L3_TABLE_CASES = [
+    ("sparse128_reference", lambda: G.disk_packing(128, 0.85, 3, 6, seed=4), False),
+    ("sparse256_torus", lambda: G.disk_packing(256, 0.8, 4, 9, seed=3), True),
+    ("disks512_torus", lambda: G.disk_packing(512, 0.8, 3, 16, seed=3), True),
+]
+
+
+@pytest.mark.parametrize("name,geom,torus", L3_TABLE_CASES, ids=[c[0] for c in L3_TABLE_CASES])
+def test_device_quadtree_tables_equal_host_three_levels(name, geom, torus):
+    """The same comparison on quadtrees of at most three levels (level
+    decision launched per level, index images per level, rows per pair of
+    levels)."""
+    from mantiis_amd.hip import quadtree_check
+    banner(f"quadtree tables device vs host, three levels: {name}")
+    pore = geom()
+    kn, _ = G.uniform_kn(pore, 0.1, MFP_DX)
+    assert quadtree_check(pore, kn, periodic_y=int(torus), multiblock=3) == 0
+
+
 def test_device_quadtree_tables_at_config_size():
     """The 1024^2 lattice of the quadtree benchmark configuration."""
     import bench
@@ -190,3 +209,87 @@ def test_device_quadtree_tables_at_config_size():
     banner("quadtree tables device vs host: 1024^2 benchmark lattice")
     pore, kn, lp, phi = bench.quadtree_inputs(1024)
     assert quadtree_check(pore, kn, periodic_y=1) == 0
+    assert quadtree_check(pore, kn, periodic_y=1, multiblock=3) == 0
+
+
+# ---------------------------------------------------------------- three levels
+def boxed128():
+    """A sparse packing inside a solid frame.  The reference's seam rule
+    (x-seam links overwritten periodically, y-seam links dropped) diverges
+    on quadtree lattices whose seams are open (the oracle too, on two levels
+    as on three), so the reference-seam case is closed on all four sides."""
+    p = G.disk_packing(128, 0.9, 3, 5, seed=7)
+    p[0, :] = p[-1, :] = p[:, 0] = p[:, -1] = 0
+    return G.sanitize(p)
+
+
+L3_CASES = [("boxed128_reference", boxed128, False),
+            ("sparse256_torus", lambda: G.disk_packing(256, 0.8, 4, 9, seed=3), True)]
+
+
+@pytest.mark.parametrize("name,geom,torus", L3_CASES, ids=[c[0] for c in L3_CASES])
+def test_three_levels_links_and_fp64_steps(name, geom, torus):
+    """A quadtree of three levels (maxAllowedLevel = 3): per coarse step one
+    level-3 step, two level-2 and four level-1 substeps, then the interfaces
+    of levels (1, 2) and (2, 3) in that order (coreLB.hpp:1310-1331), in
+    FP64 against the oracle: link lists equal, populations to 1e-15."""
+    steps = 40 if torus else 60
+    banner(f"three-level {name} FP64 links + {steps} steps")
+    pore = geom()
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    o = O.Oracle(pore, kn, lp, multiblock=3, periodic_y=torus, Fbody=1e-6)
+    g = Lattice(pore, kn, lp, precision="fp64", multiblock=3, periodic_y=int(torus), Fbody=1e-6)
+    assert g.ncell == o.ncell and int(np.max(g.grid()["level"])) == 3
+    for which in ("stream", "boundary", "periodic"):
+        a, b = o.links(which), g.links(which)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2]), which
+        if which != "boundary":
+            assert np.array_equal(a[1], b[1]), which
+    o.run("transport", steps, 0.0)
+    g.step("transport", steps)
+    d = np.abs(o.get("f") - g.get("f")).max()
+    print("max |f| diff", d, "residual", g.residual(), o.diff_history()[-1])
+    assert d <= 1e-15
+    assert abs(g.residual() - o.diff_history()[-1]) <= 1e-12 * o.diff_history()[-1]
+    ux, uy, rho = g.complete_domain()
+    oux, ouy, orho = (np.zeros(pore.size) for _ in range(3))
+    o.lib.orc_pixel_domain(o.h, oux.ctypes.data_as(O._D), ouy.ctypes.data_as(O._D), orho.ctypes.data_as(O._D))
+    assert np.array_equal(ux, oux) and np.array_equal(rho, orho)
+    o.close()
+    g.close()
+
+
+@pytest.mark.parametrize("name,geom,torus", L3_CASES, ids=[c[0] for c in L3_CASES])
+def test_three_levels_fp32_against_fp64_oracle(name, geom, torus):
+    """The FP32 three-level step (each level's fused substeps, then each
+    pair's interface as overrides; above the finest pair a child that is a
+    coarse buffer of the pair below takes the value just formed for it)
+    against the FP64 oracle, to FP32 rounding."""
+    steps = 100
+    banner(f"three-level {name} FP32 vs FP64 oracle {steps} steps")
+    pore = geom()
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    o = O.Oracle(pore, kn, lp, multiblock=3, periodic_y=torus, Fbody=1e-6)
+    g = Lattice(pore, kn, lp, precision="fp32dev", multiblock=3, periodic_y=int(torus), Fbody=1e-6)
+    o.run("transport", steps, 0.0)
+    g.step("transport", steps)
+    e = {k: rel_l2(g.get(k), o.get(k)) for k in ("rho", "ux", "uy")}
+    print("FP32 three-level rel-l2", e)
+    assert e["rho"] < 1e-9 and e["ux"] < 6e-6 and e["uy"] < 7e-6  # measured: 1e-10, 5.9e-7, 7.1e-7
+    o.close()
+    g.close()
+
+
+def test_three_levels_fp32_call_chunking_is_exact():
+    """2 x 25 coarse steps equal 50 in one call on three levels."""
+    banner("three-level sparse256 FP32 50 steps in one call vs two calls")
+    pore = G.disk_packing(256, 0.8, 4, 9, seed=3)
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    a = Lattice(pore, kn, lp, precision="fp32dev", multiblock=3, periodic_y=1, Fbody=1e-6)
+    b = Lattice(pore, kn, lp, precision="fp32dev", multiblock=3, periodic_y=1, Fbody=1e-6)
+    a.step("transport", 50)
+    b.step("transport", 25)
+    b.step("transport", 25)
+    d = np.abs(a.get("f") - b.get("f")).max()
+    print("max |f| diff", d)
+    assert d <= 1e-14

@@ -35,12 +35,12 @@ GEOMS = geometries()
 
 
 @pytest.mark.parametrize("torus", [False, True])
-@pytest.mark.parametrize("multiblock", [False, True])
+@pytest.mark.parametrize("multiblock", [0, 1, 3], ids=["single", "quadtree2", "quadtree3"])
 @pytest.mark.parametrize("geom", GEOMS, ids=[g[0] for g in GEOMS])
 def test_library_grid_equals_oracle(geom, multiblock, torus):
     name, pore, kn, lp = geom
     o = O.Oracle(pore, kn, lp, multiblock=multiblock, periodic_y=torus)
-    g = HostGrid(pore, kn, lp, multiblock=int(multiblock), periodic_y=int(torus))
+    g = HostGrid(pore, kn, lp, multiblock=multiblock, periodic_y=int(torus))
     n, L = g.size()
     assert n == o.ncell and L == o.lib.orc_max_level(o.h)
     # a multi-block grid's relaxation times come from per-level pore sizes
@@ -51,10 +51,78 @@ def test_library_grid_equals_oracle(geom, multiblock, torus):
     if not multiblock:
         assert np.array_equal(o.get("taus"), b["taus"])
     if multiblock:
-        for which in ("parent_child", "coarse", "fine"):
-            assert np.array_equal(O.oracle_rows(o, which), g.rows(which)), which
-        if name != "fixture64":
+        assert np.array_equal(O.oracle_rows(o, "parent_child"), g.rows("parent_child"))
+        for key, w in ((21, 0), (32, 0), (12, 1), (23, 1)):
+            assert np.array_equal(O.oracle_rows_key(o, w, key).reshape(-1, 5), g.rows(key)), key
+        if name != "fixture64" and multiblock == 1:
             assert L == 2 and (b["type"] == 12).sum() > 0 and (b["type"] == 21).sum() > 0
+
+
+@pytest.mark.parametrize("torus", [False, True])
+def test_three_level_grid_equals_oracle(torus):
+    """A packing sparse enough for level-3 cells (4 x 4 blocks at distance
+    >= 24 from every wall pixel; level 2 then needs distance >= 12): every
+    table and the rows of each buffer type (21, 32, 12, 23), bit for bit."""
+    pore = G.disk_packing(256, 0.8, 4, 9, seed=3)
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    o = O.Oracle(pore, kn, lp, multiblock=3, periodic_y=torus)
+    g = HostGrid(pore, kn, lp, multiblock=3, periodic_y=int(torus))
+    n, L = g.size()
+    assert n == o.ncell and L == o.lib.orc_max_level(o.h) == 3
+    a, b = o.grid(), g.arrays(taus=False)
+    for k in ("gridID", "gridIJ", "level", "connect", "bnd", "isBuffer", "type"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(O.oracle_rows(o, "parent_child"), g.rows("parent_child"))
+    for key, w in ((21, 0), (32, 0), (12, 1), (23, 1)):
+        rows = g.rows(key)
+        assert len(rows) > 0, key
+        assert np.array_equal(O.oracle_rows_key(o, w, key), rows), key
+    # the flat lists are the pairs one after the other
+    assert np.array_equal(g.rows("coarse"), np.concatenate([g.rows(21), g.rows(32)]))
+    assert np.array_equal(g.rows("fine"), np.concatenate([g.rows(12), g.rows(23)]))
+    o.close()
+
+
+def test_three_level_structure_invariants():
+    """Cells of level l >= 2 are aligned all-fluid blocks of side 2^(l-1) at
+    distance >= ceil(4 sqrt 2) 2^(l-1) = 6 * 2^(l-1) from every wall pixel
+    (minimum image on the torus), and the block that holds a level-2 cell is
+    not itself at distance >= 24; each fine row lists the four quadrants of
+    its parent."""
+    pore = G.disk_packing(256, 0.8, 4, 9, seed=3)
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    g = HostGrid(pore, kn, lp, multiblock=3, periodic_y=1)
+    a = g.arrays(taus=False)
+    walls = np.argwhere(pore == 0)
+    N = pore.shape[0]
+
+    def mind2(x, y, h):
+        px = np.arange(x, x + h).repeat(h)
+        py = np.tile(np.arange(y, y + h), h)
+        dy = np.abs(walls[:, 0][:, None] - py[None, :])
+        dx = np.abs(walls[:, 1][:, None] - px[None, :])
+        dy = np.minimum(dy, N - dy)
+        dx = np.minimum(dx, N - dx)
+        return (dx * dx + dy * dy).min()
+
+    orig = a["isBuffer"] == 0
+    for lvl in (2, 3):
+        h = 1 << (lvl - 1)
+        cells = np.nonzero((a["level"] == lvl) & orig)[0]
+        assert len(cells) > 0
+        for c in cells[:: max(1, len(cells) // 60)]:
+            x, y = a["gridIJ"][c]
+            assert x % h == 0 and y % h == 0 and pore[y:y + h, x:x + h].all()
+            assert mind2(x, y, h) >= (6 * h) ** 2
+            if lvl == 2:
+                bx, by = x - x % 4, y - y % 4
+                assert mind2(bx, by, 4) < 24 ** 2
+    for key in (12, 23):
+        for row in g.rows(key)[:200]:
+            px, py = a["gridIJ"][row[0]]
+            hj = 1 << (a["level"][row[0]] - 2)
+            kids = {tuple(a["gridIJ"][k]) for k in row[1:]}
+            assert kids == {(px, py), (px + hj, py), (px, py + hj), (px + hj, py + hj)}, key
 
 
 def test_multiblock_structure_invariants():
