@@ -25,8 +25,9 @@ FP64 single-block path (permeability error).  The CPU baseline is the oracle
 (the C++ restatement of the reference, -Ofast -fopenmp) on one lattice of the
 batch, with the host's threads and with one.
 
-Other workloads: quadtree (multi-block transport of a 1024^2 packing, a step
-being one coarse step), shanchen (Shan-Chen adsorption, 2048^2, 88 B per
+Other workloads: quadtree (multi-block transport of a 1024^2 packing, a
+quadtree of at most --levels levels, 3 by default; a step is one coarse
+step), shanchen (Shan-Chen adsorption, 2048^2, 88 B per
 update), transport (single-block FP32 transport, --n), fp64 (single-block
 FP64 transport, 256^2, 144 B per update), warmstart (the learned warm start
 against plain runs), shape (Shape::calculateProperties, 1024^2).
@@ -149,14 +150,16 @@ def run_to_steady(lat, tol, chunk, max_steps, max_s):
     return n, time.perf_counter() - t0, crit, lat.residual()
 
 
-PROFILE_FOR = {"batched": "k_cbat_batched_256.md", "quadtree": "k_cmbx_quadtree_transport_1024.md", "shanchen": "k_csc_shanchen_adsorption_2048.md",
+PROFILE_FOR = {"batched": "k_cbat_batched_256.md", "quadtree": "k_cmbx_quadtree_transport_1024.md",
+               "quadtree3": "k_cmb_quadtree3_transport_1024.md", "shanchen": "k_csc_shanchen_adsorption_2048.md",
                "transport": "k_ctr_transport_4096.md"}
 
 
 def profiled_traffic(workload, n):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 summary of the same workload (FETCH_SIZE + WRITE_SIZE scaled by
-    the copy-kernel calibration); None when there is none."""
+    """HBM bytes per launch of the dominant kernel (per coarse step, for the
+    three-level quadtree) from the committed rocprofv3 summary of the same
+    workload (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration);
+    None when there is none."""
     name = PROFILE_FOR.get(workload)
     if name is None or (workload == "transport" and n != 4096):
         return None
@@ -206,16 +209,22 @@ def time_steps(lat, kind, steps, warmup):
     return ms, kms, klaunch
 
 
-def cpu_baseline_quadtree(sample_n, sample_steps):
+def quadtree_updates(level):
+    """Cell updates of one coarse step: a cell of level l of L updates
+    2^(L - l) times (coreLB.hpp:1310-1331)."""
+    L = int(level.max())
+    return int(sum((1 << (L - l)) * int((level == l).sum()) for l in range(1, L + 1)))
+
+
+def cpu_baseline_quadtree(sample_n, sample_steps, levels):
     """Oracle multi-block transport on a sample_n^2 lattice of the same
     distribution (the reference's quadtree build is quadratic in the
     lattice size, so the 1024^2 lattice itself takes minutes to set up)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
     pore, kn, lp, phi = quadtree_inputs(sample_n)
-    o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-6, fast=True, periodic_y=True, multiblock=True)
-    lvl = o.grid()["level"]
-    upd = int(2 * (lvl == 1).sum() + (lvl == 2).sum()) if lvl.max() > 1 else int(len(lvl))
+    o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-6, fast=True, periodic_y=True, multiblock=levels)
+    upd = quadtree_updates(o.grid()["level"])
     o.run("transport", 1, 0.0)  # warm-up
     t0 = time.perf_counter()
     o.run("transport", sample_steps, 0.0)
@@ -608,6 +617,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU sample steps (default: quadtree 1000, shanchen 16)")
     ap.add_argument("--cpu-n", type=int, default=256, help="quadtree: lattice side of the CPU sample")
     ap.add_argument("--n", type=int, default=0, help="lattice side of the transport workload (default 1024)")
+    ap.add_argument("--levels", type=int, default=3, choices=[2, 3],
+                    help="quadtree: most levels of the quadtree (3: the benchmark configuration; 2: the reference's)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy (torch) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -655,19 +666,25 @@ def main():
         from mantiis_amd.hip import Lattice
         pore, kn, lp, phi = quadtree_inputs(args.n or 1024)
         lat = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
-                      multiblock=1)
+                      multiblock=args.levels)
         lvl = lat.grid()["level"]
-        n1, n2 = int((lvl == 1).sum()), int((lvl == 2).sum())
+        L = int(lvl.max())
+        nl = [int((lvl == l).sum()) for l in range(1, L + 1)]
         kind, inputs = "transport", None
-        # cell updates per coarse step: two level-1 substeps and one level-2 step
-        updates = 2 * n1 + n2
+        # cell updates per coarse step: 2^(L - l) substeps of level l
+        updates = quadtree_updates(lvl)
         bytes_per_cell = 72
-        kernel_cells = n1 + n2  # one launch: first substep of both levels
+        kernel_cells = nl[0] + nl[1] if L == 2 else updates  # two levels: one launch, first substep of both
+        counts = ", ".join(f"{c} level-{l + 1}" for l, c in enumerate(nl))
         workload = (f"quadtree multi-block transport RMRT+SRBB, {lat.nx}x{lat.ny} disk packing porosity {phi:.3f} "
-                    f"(U[0.4,0.8]), Kn=0.5, torus, FP32 deviation; {n1} level-1 and {n2} level-2 cells "
-                    "(2 level-1 substeps per coarse step)")
-        kernel = ("k_cmbx (interface of the previous coarse step + first substep of both levels, RMRT+SRBB, "
-                  "stored-field write)")
+                    f"(U[0.4,0.8]), Kn=0.5, torus, FP32 deviation; at most {args.levels} levels: {counts} cells "
+                    f"({1 << (L - 1)} level-1 substeps per coarse step)")
+        if L == 2:
+            kernel = ("k_cmbx (interface of the previous coarse step + first substep of both levels, RMRT+SRBB, "
+                      "stored-field write)")
+        else:
+            kernel = ("coarse step: k_cmb per level substep (coarsest first) + k_ifx per pair of levels "
+                      "(interface overrides)")
         dtype = "fp32 (populations)"
     elif args.workload == "transport":
         from mantiis_amd.hip import Lattice
@@ -695,10 +712,16 @@ def main():
     ms, kms, klaunch = time_steps(lat, kind, args.steps, args.warmup)
     mlups = updates * args.steps / (ms / 1e3) / 1e6
     t_kernel_ms = kms / max(klaunch, 1)
-    achieved = bytes_per_cell * kernel_cells / (t_kernel_ms / 1e3) / 1e9
+    # (the three-level quadtree step times no single launch: klaunch 0)
+    achieved = bytes_per_cell * kernel_cells / (t_kernel_ms / 1e3) / 1e9 if t_kernel_ms > 0 else 0.0
     step_achieved = bytes_per_cell * updates * args.steps / (ms / 1e3) / 1e9
     dominant = None
-    if args.workload == "quadtree":
+    if args.workload == "quadtree" and L > 2:
+        # the step's launches are timed together: the roofline is taken over
+        # the whole coarse step from the events around the K-step call
+        t_kernel_ms = ms / args.steps
+        achieved = step_achieved
+    elif args.workload == "quadtree":
         # The quadtree step is two short launches (~16 and ~32 us), for which
         # per-launch events add several us; the roofline is taken over the
         # whole coarse step from the events around the K-step call, and the
@@ -713,16 +736,26 @@ def main():
     if args.workload == "quadtree" and not args.no_steady:
         from mantiis_amd.hip import Lattice
         lat2 = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
-                       multiblock=1)
+                       multiblock=args.levels)
         st, sec, crit, r = run_to_steady(lat2, 1e-6, 100, 400000, 120.0)
         steady = {"criterion": "|d<u_x>/dt| / |<u_x>| < 1e-6 per step, checked every 100 steps", "steps": st,
                   "wall_s": round(sec, 3), "final": crit, "reference_residual": r, "plain": True}
+        # the FP64 library path for the same coarse steps: the steady
+        # permeability's FP32 error
+        lat3 = Lattice(pore, kn, lp, precision="fp64", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
+                       multiblock=args.levels)
+        lat3.step("transport", st)
+        k32, k64 = lat2.permeability()["k_lattice"], lat3.permeability()["k_lattice"]
+        steady["fp32_vs_fp64"] = {"k_fp32": k32, "k_fp64": k64, "k_rel_err": abs(k32 - k64) / abs(k64),
+                                  "ux_rel_l2": float(np.linalg.norm(lat2.get("ux") - lat3.get("ux")) /
+                                                     np.linalg.norm(lat3.get("ux")))}
+        lat3.close()
         lat2.close()
     diag = lat.diagnostics()
 
     cpu = None
     if not args.no_cpu and args.workload == "quadtree":
-        cpu = cpu_baseline_quadtree(args.cpu_n, args.cpu_steps or 1000)
+        cpu = cpu_baseline_quadtree(args.cpu_n, args.cpu_steps or (1000 if args.levels == 2 else 500), args.levels)
     elif not args.no_cpu and inputs is not None:
         cpu = cpu_baseline(inputs, args.cpu_steps or 16)
 
@@ -742,7 +775,8 @@ def main():
         "config": {"workload": workload, "lattice": [lat.nx, lat.ny], "fluid_cells": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(args.workload, args.n),
+                     "traffic": profiled_traffic(args.workload + ("3" if args.workload == "quadtree" and L > 2 else ""),
+                                                 args.n),
                      "bytes_per_cell_update": bytes_per_cell, "cells_per_launch": kernel_cells,
                      "kernel_ms": round(t_kernel_ms, 6), "launches_timed": klaunch,
                      "step_achieved": round(step_achieved, 1)},

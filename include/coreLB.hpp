@@ -80,7 +80,8 @@ public:
         p.Gff = _GLOBAL_::Gff;
         p.Gfs = _GLOBAL_::Gfs;
         p.precision = precision_from_env();
-        p.periodic_y = torus_from_env();
+        p.periodic_y = mantiis_torus_from_env();
+        p.multiblock = G.isMultigrid ? mantiis_levels_from_env() : 0;
         const std::vector<int32_t> pore = shape.flatDomain();
         const std::vector<double> kn = shape.flat(shape.Kn), lp = shape.flat(shape.LocPore);
         mts_sim h = nullptr;
@@ -225,10 +226,6 @@ private:
     static int precision_from_env() {
         const char* e = std::getenv("MANTIIS_PRECISION");
         return (e && std::string(e) == "fp32") ? MTS_FP32DEV : MTS_FP64;
-    }
-    static int torus_from_env() {
-        const char* e = std::getenv("MANTIIS_TORUS");
-        return (e && std::string(e) == "1") ? 1 : 0;
     }
     void run(int method, int lvl = 1) {
         check(mts_method(handle(), method, lvl));

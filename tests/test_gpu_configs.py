@@ -125,20 +125,23 @@ def test_batched_config_fp32_against_fp64_and_oracle():
     c.close()
 
 
-def test_quadtree_config_1024_fp32_against_fp64():
+@pytest.mark.parametrize("levels", [2, 3])
+def test_quadtree_config_1024_fp32_against_fp64(levels):
     """The quadtree configuration of the benchmark (1024^2 disk packing,
-    porosity U[0.4, 0.8], Kn = 0.5, regularised MRT + SRBB, torus): the FP32
-    step run to steady state (mean-velocity change below 1e-6 per step),
-    then the FP64 library path for the same number of coarse steps; the
-    steady permeability (FP64 report) and the fields agree."""
-    banner("quadtree 1024^2: FP32 to steady state vs FP64 at the same step count")
+    porosity U[0.4, 0.8], Kn = 0.5, regularised MRT + SRBB, torus; at most
+    two levels as the reference builds it, and three as the configuration
+    names): the FP32 step run to steady state (mean-velocity change below
+    1e-6 per step), then the FP64 library path for the same number of coarse
+    steps; the steady permeability (FP64 report) and the fields agree."""
+    banner(f"quadtree 1024^2, {levels} levels: FP32 to steady state vs FP64 at the same step count")
     pore, kn, lp, phi = bench.quadtree_inputs(1024)
     a = Lattice(pore, kn, lp, precision="fp32dev", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
-                multiblock=1)
+                multiblock=levels)
+    assert int(np.max(a.grid()["level"])) == levels
     steps, sec, crit, _ = bench.run_to_steady(a, 1e-6, 100, 100000, 120.0)
     assert crit < 1e-6, crit
     b = Lattice(pore, kn, lp, precision="fp64", collision=2, wall=2, dx=1e-9, Fbody=1e-6, periodic_y=1,
-                multiblock=1)
+                multiblock=levels)
     b.step("transport", steps)
     ka, kb = a.permeability(), b.permeability()
     err = abs(ka["k_lattice"] - kb["k_lattice"]) / abs(kb["k_lattice"])

@@ -150,3 +150,52 @@ def test_resume_from_dump_continues_exactly(tmp_path):
         # 15 significant digits of f (~0.03) carry ~1e-17 absolute error into
         # every moment; the velocities are differences of f of order 1e-6
         assert np.abs(d - c).max() <= 1e-13, field
+
+
+def write_inputs(folder, pore, kn, lp):
+    """Shape::loadExistingModel's column-major files (shape.hpp:253-285):
+    the value of row i, column j on line j N + i."""
+    np.savetxt(folder + "pore.dat", pore.T.reshape(-1), fmt="%d")
+    np.savetxt(folder + "Kn.dat", kn.T.reshape(-1), fmt="%.17g")
+    np.savetxt(folder + "localporesize.dat", lp.T.reshape(-1), fmt="%.17g")
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_app_multiblock_against_oracle(levels, tmp_path):
+    """MantiisApp with is_multiblock = true (mantiisApp.hpp:138, 166: the
+    quadtree Grid2D and the multi-block transport loop) through include/, on
+    a sparse 128^2 packing closed on all four sides; MANTIIS_LEVELS=3 allows
+    three quadtree levels.  The grid size printed at set-up and the binary
+    output fields equal the oracle's multi-block run bit for bit."""
+    from mantiis_amd import geometry as G
+    banner(f"drop-in MantiisApp multi-block ({levels} levels) 128x128 60 steps vs oracle")
+    pore = G.disk_packing(128, 0.9, 3, 5, seed=7)
+    pore[0, :] = pore[-1, :] = pore[:, 0] = pore[:, -1] = 0
+    pore = G.sanitize(pore)
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    d = tmp_path / "io"
+    d.mkdir()
+    folder = str(d) + "/"
+    write_inputs(folder, pore, kn, lp)
+    exe = build("app_driver", tmp_path)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    env.pop("MANTIIS_LEVELS", None)
+    env.pop("MANTIIS_TORUS", None)
+    if levels == 3:
+        env["MANTIIS_LEVELS"] = "3"
+    settings = ["type=transport", "verbose=1000", "dx=1e-9", "fbody=1e-6", "macro_every=1000000", "dist_first=-1",
+                "binary=1"]
+    r = subprocess.run([str(exe), *settings, "--", "1", "true", "128", folder, "60"], capture_output=True, text=True,
+                       timeout=300, env=env)
+    print(r.stdout[-400:])
+    assert r.returncode == 0, r.stderr
+    o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-6, multiblock=levels)
+    assert int(o.grid()["level"].max()) == levels
+    assert f"The number of active cells: {o.ncell}" in r.stdout
+    o.run("transport", 60, 0.0)
+    oux, ouy, orho = (np.zeros(pore.size) for _ in range(3))
+    o.lib.orc_pixel_domain(o.h, oux.ctypes.data_as(O._D), ouy.ctypes.data_as(O._D), orho.ctypes.data_as(O._D))
+    for name, want in (("ux", oux), ("uy", ouy), ("rho", orho)):
+        got = np.fromfile(folder + name + ".bin")
+        assert got.size == want.size and np.array_equal(got, want), (name, np.abs(got - want).max())
+    o.close()
