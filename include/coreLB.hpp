@@ -147,6 +147,11 @@ public:
         run(MTS_M_DENSITY_DIFF);
         record_residual();
     }
+    // coreLB.hpp:1101-1231: the buffer cells of levels l1 and l2 = l1 + 1
+    void interpolateBlockInterface(int l1, int l2) {
+        if (l2 != l1 + 1) throw std::invalid_argument("interpolateBlockInterface: levels l1 and l1 + 1");
+        run(MTS_M_INTERPOLATE, l1);
+    }
     // coreLB.hpp:1310-1331 (fb = f, collide, ftemp = f, zeroDown, stream, BC)
     void evolutionStepOfMultiBlock(int lvl) {
         const int rep = 1 << (grid.maxLevel - lvl);
@@ -159,6 +164,7 @@ public:
             applyBoundaryConditions(lvl);
         }
         if (lvl > 1) evolutionStepOfMultiBlock(lvl - 1);
+        if (lvl != grid.maxLevel) interpolateBlockInterface(lvl, lvl + 1);
     }
 
     // Fused whole steps (one HIP pass per step): n steps of the transport,

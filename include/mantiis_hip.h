@@ -220,6 +220,7 @@ int mts_residual(mts_sim s, double* diff);
 #define MTS_M_COPY_F_FB 25             /* fb = f               */
 #define MTS_M_COPY_USQ_OLD 26          /* usq_old = usq        */
 #define MTS_M_COPY_RHO_OLD 27          /* rho_old = rho        */
+#define MTS_M_INTERPOLATE 28           /* 1101-1231: interpolateBlockInterface(lvl, lvl + 1) */
 int mts_method(mts_sim s, int method, int lvl);
 
 /* Public vectors of LB2D (coreLB.hpp:89-114) in compact order.

@@ -95,6 +95,8 @@ struct mts_sim_s {
         int32_t* d_sites = nullptr;
         void* d_A = nullptr;
         double *d_f = nullptr, *d_rho = nullptr, *d_ux = nullptr, *d_uy = nullptr, *d_usq = nullptr, *d_usq_old = nullptr;
+        // arrays of the per-method calls (allocated by the first one)
+        double *d_ftemp = nullptr, *d_feq = nullptr, *d_fneq = nullptr, *d_fb = nullptr, *d_rho_old = nullptr;
     };
     bool multi = false;
     MultiGeometry mg;
@@ -945,8 +947,11 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
     return set_residual(&s->diff, s1 / s2);
 }
 
+static int multi_transport_phases(mts_sim s, int nsteps);
+
 static int multi_transport(mts_sim_s* s, int nsteps) {
     if (!s->lc.empty()) return multi_transport32(s, nsteps);
+    if (s->prm.wall == MTS_WALL_MDBB) return multi_transport_phases(s, nsteps);
     const int Lmax = (int)s->lv.size();
     int rc;
     for (int step = 0; step < nsteps; step++) {
@@ -1001,13 +1006,22 @@ static int multi_transport(mts_sim_s* s, int nsteps) {
 }
 
 // Downloads one per-cell field of a multi-block lattice into compact order.
+static int multi_ensure_phase(mts_sim s);
+
 static int multi_get(mts_sim_s* s, int field, double* out) {
-    const int q = field == MTS_F_F ? 9 : 1;
+    const bool pop = field == MTS_F_F || field == MTS_F_FTEMP || field == MTS_F_FEQ || field == MTS_F_FNEQ ||
+                     field == MTS_F_FB;
+    const int q = pop ? 9 : 1;
+    int rc;
+    if ((field == MTS_F_FTEMP || field == MTS_F_FEQ || field == MTS_F_FNEQ || field == MTS_F_FB ||
+         field == MTS_F_RHO_OLD) && (rc = multi_ensure_phase(s)))
+        return rc;
     for (size_t l = 0; l < s->lv.size(); l++) {
         auto& L = s->lv[l];
         const double* src = field == MTS_F_F ? L.d_f : field == MTS_F_RHO ? L.d_rho : field == MTS_F_UX ? L.d_ux
                           : field == MTS_F_UY ? L.d_uy : field == MTS_F_USQ ? L.d_usq : field == MTS_F_USQ_OLD ? L.d_usq_old
-                                                                                                                 : nullptr;
+                          : field == MTS_F_FTEMP ? L.d_ftemp : field == MTS_F_FEQ ? L.d_feq : field == MTS_F_FNEQ ? L.d_fneq
+                          : field == MTS_F_FB ? L.d_fb : field == MTS_F_RHO_OLD ? L.d_rho_old : nullptr;
         if (!src) return fail(MTS_ERR_UNSUPPORTED, "field not held by multi-block lattices");
         std::vector<double> h((size_t)(q == 9 ? L.stride * 9 : L.P));
         HIP_TRY(hipMemcpyAsync(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
@@ -1104,9 +1118,9 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
     }
     std::string err;
     if (prm.multiblock) {
-        if (prm.wall == MTS_WALL_MDBB) {
+        if (prm.wall == MTS_WALL_MDBB && prm.precision == MTS_FP32DEV) {
             delete s;
-            return fail(MTS_ERR_UNSUPPORTED, "MDBB walls are not provided on multi-block lattices");
+            return fail(MTS_ERR_UNSUPPORTED, "MDBB walls on multi-block lattices run in FP64 (precision MTS_FP64)");
         }
         int rc = create_multi(s, nx, ny, pore, kn, locpore, err);
         if (rc) {
@@ -1563,9 +1577,150 @@ int mts_residual(mts_sim s, double* diff) {
     return MTS_OK;
 }
 
+// ---- per-method calls on a multi-block lattice: each level's dense site
+// image takes the single-block kernels.  A method with a level
+// (coreLB.hpp:678-977) runs on that level's image; a method over every cell
+// runs on every image; the regularised collision of one level first forms
+// fneq of the others (regularize() runs over all cells, coreLB.hpp:744-792).
+static int multi_ensure_phase(mts_sim s) {
+    int rc;
+    for (auto& V : s->lv) {
+        if (V.d_ftemp) continue;
+        const size_t np = (size_t)V.stride * 9;
+        if ((rc = s->alloc(&V.d_ftemp, np)) || (rc = s->alloc(&V.d_feq, np)) || (rc = s->alloc(&V.d_fneq, np)) ||
+            (rc = s->alloc(&V.d_fb, np)) || (rc = s->alloc(&V.d_rho_old, (size_t)V.P)))
+            return rc;
+        HIP_TRY(hipMemsetAsync(V.d_ftemp, 0, np * 8, s->stream));
+        HIP_TRY(hipMemsetAsync(V.d_feq, 0, np * 8, s->stream));
+        HIP_TRY(hipMemsetAsync(V.d_fneq, 0, np * 8, s->stream));
+        HIP_TRY(hipMemsetAsync(V.d_fb, 0, np * 8, s->stream));
+        HIP_TRY(hipMemsetAsync(V.d_rho_old, 0, (size_t)V.P * 8, s->stream));
+    }
+    return MTS_OK;
+}
+
+static int multi_method(mts_sim s, int method, int lvl) {
+    const int L = (int)s->lv.size();
+    int rc;
+    if ((rc = multi_ensure_phase(s))) return rc;
+    auto arrays = [&](int l) {
+        auto& V = s->lv[l];
+        return PhaseArrays{V.d_f, V.d_ftemp, V.d_feq, V.d_fneq, V.d_fb, V.d_rho, V.d_rho_old, V.d_ux, V.d_uy,
+                           V.d_usq, V.d_usq_old, nullptr, nullptr, nullptr};
+    };
+    auto on = [&](int l, int m) -> int {
+        const int r = launch_phase(s->lctx(l), m, 1, arrays(l), s->d_neg);
+        return r ? fail(r, "unknown method id") : s->launched();
+    };
+    using LevelDev = mts_sim_s::LevelDev;
+    auto copy = [&](double* LevelDev::*dst, double* LevelDev::*src, bool pop) -> int {
+        for (auto& V : s->lv)
+            HIP_TRY(hipMemcpyAsync(V.*dst, V.*src, (pop ? (size_t)V.stride * 9 : (size_t)V.P) * 8,
+                                   hipMemcpyDeviceToDevice, s->stream));
+        return MTS_OK;
+    };
+    switch (method) {
+        case MTS_M_COPY_F_FTEMP: return copy(&LevelDev::d_ftemp, &LevelDev::d_f, true);
+        case MTS_M_COPY_F_FB: return copy(&LevelDev::d_fb, &LevelDev::d_f, true);
+        case MTS_M_COPY_USQ_OLD: return copy(&LevelDev::d_usq_old, &LevelDev::d_usq, false);
+        case MTS_M_COPY_RHO_OLD: return copy(&LevelDev::d_rho_old, &LevelDev::d_rho, false);
+        case MTS_M_VELOCITY_DIFF: {  // sums over the cells of every level
+            double s1 = 0, s2 = 0;
+            for (int l = 0; l < L; l++) {
+                double r[4];
+                launch_reduce(s->lctx(l), 0, s->lv[l].d_usq, s->lv[l].d_usq_old, nullptr, s->d_part, s->nparts, s->d_red);
+                if ((rc = s->launched())) return rc;
+                HIP_TRY(hipMemcpyAsync(r, s->d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+                HIP_TRY(hipStreamSynchronize(s->stream));
+                s1 += r[0];
+                s2 += r[1];
+            }
+            s->diff = s1 / s2;
+            return MTS_OK;
+        }
+        case MTS_M_DENSITY_DIFF: {  // the largest over every level
+            double m = 0;
+            for (int l = 0; l < L; l++) {
+                double r[4];
+                launch_reduce(s->lctx(l), 1, s->lv[l].d_rho, s->lv[l].d_rho_old, nullptr, s->d_part, s->nparts, s->d_red);
+                if ((rc = s->launched())) return rc;
+                HIP_TRY(hipMemcpyAsync(r, s->d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+                HIP_TRY(hipStreamSynchronize(s->stream));
+                m = std::max(m, r[0]);
+            }
+            s->diff = m < 1.0e-15 ? 1.0e-5 : m;
+            return MTS_OK;
+        }
+        case MTS_M_INTERPOLATE: {  // interpolateBlockInterface(lvl, lvl + 1)
+            if (lvl < 1 || lvl >= L) return fail(MTS_ERR_ARG, "interpolateBlockInterface: no level pair (lvl, lvl + 1)");
+            const int p = lvl - 1;
+            auto &L1 = s->lv[p], &L2 = s->lv[p + 1];
+            launch_coarse_interp(s->stream, s->crow_off[p + 1] - s->crow_off[p], s->d_crows + s->crow_off[p], L2.d_f,
+                                 L2.stride, L1.d_f, L1.stride, L2.d_rho, L2.d_ux, L2.d_uy, L2.d_usq);
+            launch_fine_interp(s->stream, s->frow_off[p + 1] - s->frow_off[p], s->d_frows + s->frow_off[p], L1.d_f,
+                               L1.stride, L2.d_f, L2.stride, L1.d_rho, L1.d_ux, L1.d_uy, L1.d_usq);
+            return s->launched();
+        }
+        case MTS_M_EVOLUTION: {  // coreLB.hpp:1310-1331, with the recursion and the interfaces
+            if (lvl < 1 || lvl > L) return MTS_OK;
+            for (int i = 0; i < (1 << (L - lvl)); i++)
+                if ((rc = multi_method(s, MTS_M_COPY_F_FB, lvl)) || (rc = multi_method(s, MTS_M_COLLIDE, lvl)) ||
+                    (rc = multi_method(s, MTS_M_COPY_F_FTEMP, lvl)) || (rc = multi_method(s, MTS_M_ZERODOWN, lvl)) ||
+                    (rc = multi_method(s, MTS_M_STREAM, lvl)) || (rc = multi_method(s, MTS_M_APPLY_BC, lvl)))
+                    return rc;
+            if (lvl > 1 && (rc = multi_method(s, MTS_M_EVOLUTION, lvl - 1))) return rc;
+            if (lvl != L && (rc = multi_method(s, MTS_M_INTERPOLATE, lvl))) return rc;
+            return MTS_OK;
+        }
+        case MTS_M_EQUILIBRIUM:
+        case MTS_M_REGULARIZE:
+        case MTS_M_MACROSCOPIC:
+            for (int l = 0; l < L; l++)
+                if ((rc = on(l, method))) return rc;
+            return MTS_OK;
+        case MTS_M_COLLIDE:
+            return multi_method(s, s->prm.collision == MTS_COLL_RMRT ? MTS_M_RMRT
+                                   : s->prm.collision == MTS_COLL_MRT ? MTS_M_MRT : MTS_M_SRT, lvl);
+        case MTS_M_APPLY_BC:
+            if ((rc = multi_method(s, s->prm.wall == MTS_WALL_SRBB ? MTS_M_SRBBWALL
+                                      : s->prm.wall == MTS_WALL_MDBB ? MTS_M_MDBBWALL : MTS_M_BBWALL, lvl)))
+                return rc;
+            return multi_method(s, MTS_M_PERIODIC, lvl);
+        case MTS_M_SRT:
+        case MTS_M_MRT:
+        case MTS_M_RMRT:
+        case MTS_M_ZERODOWN:
+        case MTS_M_STREAM:
+        case MTS_M_BBWALL:
+        case MTS_M_SRBBWALL:
+        case MTS_M_MDBBWALL:
+        case MTS_M_PERIODIC:
+            if (lvl < 1 || lvl > L) return MTS_OK;  // no cell has that level
+            if (method == MTS_M_RMRT)
+                for (int l = 0; l < L; l++)
+                    if (l != lvl - 1 && (rc = on(l, MTS_M_REGULARIZE))) return rc;
+            return on(lvl - 1, method);
+        default:
+            return fail(MTS_ERR_UNSUPPORTED, "multi-block lattices: the adsorption methods are single-block only");
+    }
+}
+
+// The transport loop of mantiisApp.hpp:239-246 method by method (operators
+// without a fused multi-block kernel: MDBB walls).
+static int multi_transport_phases(mts_sim s, int nsteps) {
+    const int L = (int)s->lv.size();
+    int rc;
+    for (int i = 0; i < nsteps; i++)
+        if ((rc = multi_method(s, MTS_M_COPY_USQ_OLD, 1)) || (rc = multi_method(s, MTS_M_EQUILIBRIUM, 1)) ||
+            (rc = multi_method(s, MTS_M_EVOLUTION, L)) || (rc = multi_method(s, MTS_M_MACROSCOPIC, 1)) ||
+            (rc = multi_method(s, MTS_M_VELOCITY_DIFF, 1)))
+            return rc;
+    return set_residual(&s->diff, s->diff);
+}
+
 int mts_method(mts_sim s, int method, int lvl) {
     if (!s) return fail(MTS_ERR_ARG, "null handle");
-    if (s->multi) return fail(MTS_ERR_UNSUPPORTED, "multi-block lattices provide the fused transport step only");
+    if (s->multi) return multi_method(s, method, lvl);
     int rc;
     if ((rc = s->ensure_phase()) || (rc = s->ensure_sc())) return rc;
     const size_t np = (size_t)s->geo.stride * 9 * 8;

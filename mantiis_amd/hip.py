@@ -22,7 +22,7 @@ METHODS = {
     "getPseudoPotential": 17, "calculateForces": 18, "applySourceTerm": 19,
     "calculateVelocityDifference": 20, "calculateDensityDifference": 21, "applyBoundaryConditions": 22,
     "evolutionStepOfMultiBlock": 23, "copy_f_to_ftemp": 24, "copy_f_to_fb": 25, "copy_usq_to_usq_old": 26,
-    "copy_rho_to_rho_old": 27,
+    "copy_rho_to_rho_old": 27, "interpolateBlockInterface": 28,
 }
 FIELDS = {"f": 0, "ftemp": 1, "feq": 2, "fneq": 3, "fb": 4, "rho": 5, "rho_old": 6, "ux": 7, "uy": 8,
           "usq": 9, "usq_old": 10, "psi": 11, "Fx": 12, "Fy": 13, "taus": 14, "kn": 15}

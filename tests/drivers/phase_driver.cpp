@@ -2,7 +2,7 @@
 // MantiisApp::transport_simulation_loop does (mantiisApp.hpp:239-246):
 // usq_old = usq, equilibrium, evolutionStepOfMultiBlock,
 // calculateMacroscopicPorperties, calculateVelocityDifference.
-// argv: <Nx> <input_folder/> <Nt> <out_prefix>
+// argv: <Nx> <input_folder/> <Nt> <out_prefix> [<is_multiblock>]
 #include <iostream>
 #include "include/mantiisApp.hpp"
 
@@ -12,13 +12,14 @@ int main(int argc, char* argv[]) {
     std::string folder = argv[2];
     const int iters = std::stoi(argv[3]);
     const std::string out = argv[4];
+    const bool multi = argc > 5 && std::string(argv[5]) == "true";
     _GLOBAL_::Cl = 1.0e-9;
     _GLOBAL_::Fbody = 1.0e-8;
     _GLOBAL_::recompute();
     lattice latt;
     Shape shape(N, N, _GLOBAL_::Cl);
     shape.loadExistingModel(folder);
-    Grid2D grid(shape, latt, false);
+    Grid2D grid(shape, latt, multi);
     grid.initialize(shape);
     int nx = N, ny = N;
     LB2D lb(nx, ny, latt, grid, shape);

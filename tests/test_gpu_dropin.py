@@ -199,3 +199,38 @@ def test_app_multiblock_against_oracle(levels, tmp_path):
         got = np.fromfile(folder + name + ".bin")
         assert got.size == want.size and np.array_equal(got, want), (name, np.abs(got - want).max())
     o.close()
+
+
+def test_method_by_method_driver_multiblock(tmp_path):
+    """The method-by-method LB2D driver on a three-level quadtree
+    (is_multiblock, MANTIIS_LEVELS=3): evolutionStepOfMultiBlock recurses
+    over the levels and interpolates each pair of levels through the C ABI;
+    the written fields (15 significant digits) equal the oracle's."""
+    from mantiis_amd import geometry as G
+    banner("drop-in LB2D method-by-method driver, three-level quadtree 128x128 30 steps")
+    pore = G.disk_packing(128, 0.9, 3, 5, seed=7)
+    pore[0, :] = pore[-1, :] = pore[:, 0] = pore[:, -1] = 0
+    pore = G.sanitize(pore)
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    d = tmp_path / "io"
+    d.mkdir()
+    folder = str(d) + "/"
+    write_inputs(folder, pore, kn, lp)
+    exe = build("phase_driver", tmp_path)
+    env = dict(os.environ, MANTIIS_LEVELS="3")
+    env.pop("MANTIIS_TORUS", None)
+    r = subprocess.run([str(exe), "128", folder, "30", folder + "m_", "true"], capture_output=True, text=True,
+                       timeout=300, env=env)
+    print(r.stdout[-300:])
+    assert r.returncode == 0, r.stderr
+    o = O.Oracle(pore, kn, lp, dx=1e-9, Fbody=1e-8, multiblock=3)
+    assert int(o.grid()["level"].max()) == 3
+    o.run("transport", 30, 0.0)
+    oux, ouy, orho = (np.zeros(pore.size) for _ in range(3))
+    o.lib.orc_pixel_domain(o.h, oux.ctypes.data_as(O._D), ouy.ctypes.data_as(O._D), orho.ctypes.data_as(O._D))
+    for name, want in (("ux", oux), ("uy", ouy), ("rho", orho)):
+        got = np.loadtxt(folder + "m_" + name + ".txt").reshape(-1)
+        assert np.allclose(got, want, rtol=1e-13, atol=1e-300), name
+    conv = np.loadtxt(folder + "m_convergence.txt")
+    assert abs(conv[-1] - o.diff_history()[-1]) <= 1e-13 * abs(o.diff_history()[-1])
+    o.close()

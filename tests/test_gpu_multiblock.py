@@ -293,3 +293,76 @@ def test_three_levels_fp32_call_chunking_is_exact():
     d = np.abs(a.get("f") - b.get("f")).max()
     print("max |f| diff", d)
     assert d <= 1e-14
+
+
+# ------------------------------------------------- per-method calls, MDBB
+MB_METHOD_CASES = [("boxed128_reference_2", boxed128, False, 2), ("boxed128_reference_3", boxed128, False, 3),
+                   ("sparse128_torus_3", lambda: G.disk_packing(128, 0.85, 3, 6, seed=4), True, 3)]
+
+
+@pytest.mark.parametrize("name,geom,torus,levels", MB_METHOD_CASES, ids=[c[0] for c in MB_METHOD_CASES])
+def test_multiblock_method_by_method_against_oracle(name, geom, torus, levels):
+    """The transport loop of mantiisApp.hpp:239-246 called method by method
+    on a multi-block lattice (equilibrium, evolutionStepOfMultiBlock(maxLevel)
+    -- its level substeps and interfaces -- moments, residual), FP64, against
+    the oracle's loop: populations and residual bit for bit.  Then one
+    regularised collision of a single level: f of every level and fneq
+    (formed over all cells, coreLB.hpp:744-792) equal the oracle's."""
+    steps = 30
+    banner(f"multi-block {name}: {steps} transport steps method by method vs oracle")
+    pore = geom()
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    o = O.Oracle(pore, kn, lp, multiblock=levels, periodic_y=torus, Fbody=1e-6)
+    g = Lattice(pore, kn, lp, precision="fp64", multiblock=levels, periodic_y=int(torus), Fbody=1e-6)
+    L = int(np.max(g.grid()["level"]))
+    assert L == levels
+    for _ in range(steps):
+        g.method("copy_usq_to_usq_old")
+        g.method("equilibrium")
+        g.method("evolutionStepOfMultiBlock", L)
+        g.method("calculateMacroscopicPorperties")
+        g.method("calculateVelocityDifference")
+    o.run("transport", steps, 0.0)
+    d = np.abs(o.get("f") - g.get("f")).max()
+    print("max |f| diff", d, "residual", g.residual(), o.diff_history()[-1])
+    assert d <= 1e-15
+    # the residual's sums run per level image, in another order than the oracle's
+    assert abs(g.residual() - o.diff_history()[-1]) <= 1e-12 * o.diff_history()[-1]
+    for lib, orc in ((g, o),):
+        lib.method("equilibrium")
+        orc.method("equilibrium")
+        lib.method("MRTCollisionRegularized", 2)
+        orc.method("MRTCollisionRegularized", 2)
+    for field in ("f", "fneq", "feq"):
+        assert np.array_equal(g.get(field), o.get(field)), field
+    o.close()
+    g.close()
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_multiblock_mdbb_against_oracle(levels):
+    """MDBB walls (coreLB.hpp:936-967) on a quadtree lattice: the transport
+    step runs method by method (no fused multi-block kernel takes MDBB), FP64
+    against the oracle with the same operators, bit for bit."""
+    banner(f"multi-block boxed128 {levels} levels MDBB FP64 vs oracle 30 steps")
+    pore = boxed128()
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    o = O.Oracle(pore, kn, lp, multiblock=levels, Fbody=1e-6)
+    o.set_operators(COLL["RMRT"], WALL["MDBB"])
+    g = Lattice(pore, kn, lp, precision="fp64", multiblock=levels, Fbody=1e-6, collision="RMRT", wall="MDBB")
+    o.run("transport", 30, 0.0)
+    g.step("transport", 30)
+    d = np.abs(o.get("f") - g.get("f")).max()
+    print("max |f| diff", d, "residual", g.residual(), o.diff_history()[-1])
+    assert d <= 1e-15 and abs(g.residual() - o.diff_history()[-1]) <= 1e-12 * o.diff_history()[-1]
+    o.close()
+    g.close()
+
+
+def test_multiblock_fp32_mdbb_is_refused():
+    from mantiis_amd.hip import MtsError
+    pore = boxed128()
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    with pytest.raises(MtsError) as e:
+        Lattice(pore, kn, lp, precision="fp32dev", multiblock=3, wall="MDBB")
+    assert "FP64" in str(e.value)

@@ -233,3 +233,29 @@ def test_fp32_mdbb_against_fp64_oracle(coll):
     assert e["rho"] < 1e-9 and e["ux"] < 1e-5 and e["uy"] < 1e-5, e  # measured: at most 8.9e-11, 7.9e-7, 8.8e-7
     g.close()
     h.close()
+
+
+@pytest.mark.parametrize("torus", [False, True])
+def test_method_by_method_single_block_against_oracle(torus):
+    """The transport loop called method by method (mantiisApp.hpp:239-246)
+    on a single-block disk packing, under the reference's seam rule and on a
+    torus (where links across the x-seam stream like interior links and the
+    periodic step has nothing to do), FP64 against the oracle, bit for bit."""
+    banner(f"single-block disks96 method by method, torus={torus}")
+    pore = G.disk_packing(96, 0.7, 3, 8, seed=12)
+    kn, lp = G.uniform_kn(pore, 0.1, 0.7730846760658538)
+    o = O.Oracle(pore, kn, lp, periodic_y=torus, Fbody=1e-6)
+    g = Lattice(pore, kn, lp, precision="fp64", periodic_y=int(torus), Fbody=1e-6)
+    for _ in range(40):
+        g.method("copy_usq_to_usq_old")
+        g.method("equilibrium")
+        g.method("evolutionStepOfMultiBlock", 1)
+        g.method("calculateMacroscopicPorperties")
+        g.method("calculateVelocityDifference")
+    o.run("transport", 40, 0.0)
+    d = np.abs(o.get("f") - g.get("f")).max()
+    print("max |f| diff", d, "residual", g.residual(), o.diff_history()[-1])
+    assert d <= 1e-15
+    assert abs(g.residual() - o.diff_history()[-1]) <= 1e-12 * o.diff_history()[-1]
+    o.close()
+    g.close()
