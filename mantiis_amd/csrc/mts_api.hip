@@ -877,31 +877,34 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
             if ((rc = s->launched())) return rc;
             continue;
         }
-        // three levels or more: each level's substeps, coarsest first, then
-        // the interface of each pair of levels, finest first
-        // (coreLB.hpp:1310-1331), written as overrides
-        for (int l = Lmax - 1; l >= 0; l--) {
-            auto& C = s->lc[l];
-            auto& L = s->lv[l];
-            LaunchCtx c = s->lctx(l);
-            c.cimg = &C.ovr;
-            MbSpec a;
-            a.Ain = C.A[cur[l]];
-            a.Aout = C.A[cur[l] ^ 1];
-            a.fcan = L.d_f;
-            a.m = C.m[0];
-            a.usq_old = last ? L.d_usq_old : nullptr;
-            if ((rc = launch_cmultistep(c, first ? 0 : 1, 0, a))) return fail(rc, "operator combination has no kernel");
-            cur[l] ^= 1;
-            for (int rep = 1; rep < (1 << (Lmax - 1 - l)); rep++) {
-                c.cimg = &C.plain;
-                MbSpec b;
-                b.Ain = C.A[cur[l]];
-                b.Aout = C.A[cur[l] ^ 1];
-                b.m = C.m[0];
-                launch_cmultistep(c, 1, 1, b);
+        // three levels: level l does 2^(L-1-l) substeps (coreLB.hpp:1310-1331);
+        // the levels do not meet before the interfaces, so substep r of every
+        // level that has one runs in one launch: the first substeps of all
+        // levels (fresh fields, overrides), then level 1 and 2's second, then
+        // level 1's third and fourth.  Then the interface of each pair of
+        // levels, finest first, written as overrides.
+        for (int r = 0; r < (1 << (Lmax - 1)); r++) {
+            LaunchCtx c[MTS_MAX_LEVELS];
+            MbSpec a[MTS_MAX_LEVELS];
+            int n = 0;
+            for (int l = 0; l < Lmax; l++) {
+                if (r >= (1 << (Lmax - 1 - l))) continue;
+                auto& C = s->lc[l];
+                c[n] = s->lctx(l);
+                c[n].cimg = r == 0 ? &C.ovr : &C.plain;
+                a[n].Ain = C.A[cur[l]];
+                a[n].Aout = C.A[cur[l] ^ 1];
+                a[n].m = C.m[0];
+                if (r == 0) {
+                    a[n].fcan = s->lv[l].d_f;
+                    a[n].usq_old = last ? s->lv[l].d_usq_old : nullptr;
+                }
                 cur[l] ^= 1;
+                n++;
             }
+            const int mode = r == 0 && first ? 0 : 1, mac = r == 0 ? 0 : 1;
+            if ((rc = n == 1 ? launch_cmultistep(c[0], mode, mac, a[0]) : launch_cmultistep3(n, c, a, mode, mac)))
+                return fail(rc, "operator combination has no kernel");
         }
         for (int p = 0; p + 1 < Lmax; p++) {
             LaunchCtx c1 = s->lctx(p), c2 = s->lctx(p + 1);

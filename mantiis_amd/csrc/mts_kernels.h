@@ -154,6 +154,9 @@ int launch_cshanchen(const LaunchCtx& c, int kind, int mode, const ScSpec& s);
 // mode 0 from the canonical state, 1 gather; mac 0 fresh fields, 1 stored
 int launch_cmultistep(const LaunchCtx& c, int mode, int mac, const MbSpec& s);
 int launch_cinterface(const LaunchCtx& c1, const LaunchCtx& c2, const IfSpec& s);
+// Substeps of n = 2 or 3 level images in one launch (c[0] gives the
+// operators), all with the same mode and stored-field use (mac).
+int launch_cmultistep3(int n, const LaunchCtx* c, const MbSpec* s, int mode, int mac);
 // Batched single-block lattices (cell order): one block of 256 cells of one
 // lattice per queue entry.
 // A 256-cell block of the batched image: cells [start, end) of lattice
