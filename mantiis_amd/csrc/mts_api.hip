@@ -813,8 +813,9 @@ static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const
 // levels, from the second step of a call on, the interface of the previous
 // step runs in the launch of the step's first substeps and updates the
 // buffer cells itself, and the last step's interface writes the overrides;
-// on three, each pair's interface writes overrides after every step.  A call
-// starts from and ends in the canonical FP64 level images.
+// on three, the interfaces write overrides, the (2, 3) one in the next step's
+// first launch.  A call starts from and ends in the canonical FP64 level
+// images.
 static int multi_transport32(mts_sim_s* s, int nsteps) {
     const int Lmax = (int)s->lv.size();
     int rc;
@@ -877,18 +878,22 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
             if ((rc = s->launched())) return rc;
             continue;
         }
-        // three levels: level l does 2^(L-1-l) substeps (coreLB.hpp:1310-1331);
-        // the levels do not meet before the interfaces, so substep r of every
-        // level that has one runs in one launch: the first substeps of all
-        // levels (fresh fields, overrides), then level 1 and 2's second, then
-        // level 1's third and fourth.  Then the interface of each pair of
-        // levels, finest first, written as overrides.
-        for (int r = 0; r < (1 << (Lmax - 1)); r++) {
+        // three levels: level l does 2^(L-1-l) substeps (coreLB.hpp:1310-1331),
+        // then the interfaces of levels (1, 2) and (2, 3).  The levels do not
+        // meet between interfaces, so launch g runs substep g of level 1 with
+        // the substeps of levels 2 and 3 that can go beside it.  Level 1 does
+        // not read the (2, 3) interface: after the first step of a call, the
+        // previous step's (2, 3) interface runs in launch 0 beside level 1's
+        // first substep, and levels 2 and 3 start one launch later -- five
+        // launches per coarse step (the (1, 2) interface the fifth).
+        const bool pending = !first && Lmax == 3;
+        for (int g = 0; g < (1 << (Lmax - 1)); g++) {
             LaunchCtx c[MTS_MAX_LEVELS];
             MbSpec a[MTS_MAX_LEVELS];
-            int n = 0;
+            int n = 0, mac = 0;
             for (int l = 0; l < Lmax; l++) {
-                if (r >= (1 << (Lmax - 1 - l))) continue;
+                const int r = pending && l > 0 ? g - 1 : g;  // substep of level l in launch g
+                if (r < 0 || r >= (1 << (Lmax - 1 - l))) continue;
                 auto& C = s->lc[l];
                 c[n] = s->lctx(l);
                 c[n].cimg = r == 0 ? &C.ovr : &C.plain;
@@ -898,21 +903,36 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
                 if (r == 0) {
                     a[n].fcan = s->lv[l].d_f;
                     a[n].usq_old = last ? s->lv[l].d_usq_old : nullptr;
+                } else {
+                    mac |= 1 << n;
                 }
                 cur[l] ^= 1;
                 n++;
             }
-            const int mode = r == 0 && first ? 0 : 1, mac = r == 0 ? 0 : 1;
-            if ((rc = n == 1 ? launch_cmultistep(c[0], mode, mac, a[0]) : launch_cmultistep3(n, c, a, mode, mac)))
-                return fail(rc, "operator combination has no kernel");
+            const int mode = g == 0 && first ? 0 : 1;
+            if (g == 0 && pending) {  // the previous step's (2, 3) interface beside level 1's first substep
+                LaunchCtx c1 = s->lctx(1), c2 = s->lctx(2);
+                c1.cimg = &s->lc[1].ovr;
+                c2.cimg = &s->lc[2].ovr;
+                rc = launch_cinterface_beside(c1, c2, iface(1, 0), c[0], a[0], mode, mac & 1);
+            } else {
+                rc = n == 1 ? launch_cmultistep(c[0], mode, mac & 1, a[0]) : launch_cmultistep3(n, c, a, mode, mac);
+            }
+            if (rc) return fail(rc, "operator combination has no kernel");
         }
-        for (int p = 0; p + 1 < Lmax; p++) {
-            LaunchCtx c1 = s->lctx(p), c2 = s->lctx(p + 1);
-            c1.cimg = &s->lc[p].ovr;
-            c2.cimg = &s->lc[p + 1].ovr;
-            if ((rc = launch_cinterface(c1, c2, iface(p, 0)))) return fail(rc, "interface has no kernel");
+        if (Lmax > 1) {  // this step's (1, 2) interface
+            LaunchCtx c1 = s->lctx(0), c2 = s->lctx(1);
+            c1.cimg = &s->lc[0].ovr;
+            c2.cimg = &s->lc[1].ovr;
+            if ((rc = launch_cinterface(c1, c2, iface(0, 0)))) return fail(rc, "interface has no kernel");
         }
         if ((rc = s->launched())) return rc;
+    }
+    if (Lmax == 3) {  // the last step's (2, 3) interface, as overrides for the export
+        LaunchCtx c1 = s->lctx(1), c2 = s->lctx(2);
+        c1.cimg = &s->lc[1].ovr;
+        c2.cimg = &s->lc[2].ovr;
+        if ((rc = launch_cinterface(c1, c2, iface(1, 0)))) return fail(rc, "interface has no kernel");
     }
     if (Lmax == 2) {  // the last step's interface, as overrides for the export
         LaunchCtx c1 = s->lctx(0), c2 = s->lctx(1);

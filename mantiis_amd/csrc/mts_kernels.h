@@ -155,8 +155,13 @@ int launch_cshanchen(const LaunchCtx& c, int kind, int mode, const ScSpec& s);
 int launch_cmultistep(const LaunchCtx& c, int mode, int mac, const MbSpec& s);
 int launch_cinterface(const LaunchCtx& c1, const LaunchCtx& c2, const IfSpec& s);
 // Substeps of n = 2 or 3 level images in one launch (c[0] gives the
-// operators), all with the same mode and stored-field use (mac).
+// operators), all with the same mode; bit i of mac: image i uses the step's
+// stored fields.
 int launch_cmultistep3(int n, const LaunchCtx* c, const MbSpec* s, int mode, int mac);
+// An interface of a pair of levels written as overrides, and in the same
+// launch a substep of one level image that reads none of them.
+int launch_cinterface_beside(const LaunchCtx& c1, const LaunchCtx& c2, const IfSpec& f, const LaunchCtx& ca,
+                             const MbSpec& sa, int mode, int mac);
 // Batched single-block lattices (cell order): one block of 256 cells of one
 // lattice per queue entry.
 // A 256-cell block of the batched image: cells [start, end) of lattice
