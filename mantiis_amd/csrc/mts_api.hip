@@ -813,9 +813,8 @@ static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const
 // levels, from the second step of a call on, the interface of the previous
 // step runs in the launch of the step's first substeps and updates the
 // buffer cells itself, and the last step's interface writes the overrides;
-// on three, the interfaces write overrides, the (2, 3) one in the next step's
-// first launch.  A call starts from and ends in the canonical FP64 level
-// images.
+// on three, both interfaces of a step run in the next step's first two
+// launches.  A call starts from and ends in the canonical FP64 level images.
 static int multi_transport32(mts_sim_s* s, int nsteps) {
     const int Lmax = (int)s->lv.size();
     int rc;
@@ -880,26 +879,36 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
         }
         // three levels: level l does 2^(L-1-l) substeps (coreLB.hpp:1310-1331),
         // then the interfaces of levels (1, 2) and (2, 3).  The levels do not
-        // meet between interfaces, so launch g runs substep g of level 1 with
-        // the substeps of levels 2 and 3 that can go beside it.  Level 1 does
-        // not read the (2, 3) interface: after the first step of a call, the
-        // previous step's (2, 3) interface runs in launch 0 beside level 1's
-        // first substep, and levels 2 and 3 start one launch later -- five
-        // launches per coarse step (the (1, 2) interface the fifth).
+        // meet between interfaces; level 1 reads only the (1, 2) interface and
+        // levels 2 and 3 read both.  After the first step of a call, launch g
+        // runs level 1's substep g and:
+        //   g = 0: the previous step's (1, 2) interface, whose fine values
+        //          update their cells (their first substep) and whose coarse
+        //          values are written as overrides;
+        //   g = 1: the previous step's (2, 3) interface, as overrides;
+        //   g = 2: the first substeps of levels 2 and 3;
+        //   g = 3: level 2's second substep --
+        // four launches per coarse step.  The interfaces read the previous
+        // step's stored fields while the substeps write the step's own (two
+        // sets, mi).  The first step of a call runs substep g of every level
+        // that has one in launch g.
         const bool pending = !first && Lmax == 3;
         for (int g = 0; g < (1 << (Lmax - 1)); g++) {
+            // the interface specs read the carried states before this launch
+            const IfSpec f12 = pending && g == 0 ? iface(0, mi ^ 1) : IfSpec{};
+            const IfSpec f23 = pending && g == 1 ? iface(1, mi ^ 1) : IfSpec{};
             LaunchCtx c[MTS_MAX_LEVELS];
             MbSpec a[MTS_MAX_LEVELS];
             int n = 0, mac = 0;
             for (int l = 0; l < Lmax; l++) {
-                const int r = pending && l > 0 ? g - 1 : g;  // substep of level l in launch g
+                const int r = pending && l > 0 ? g - 2 : g;  // substep of level l in launch g
                 if (r < 0 || r >= (1 << (Lmax - 1 - l))) continue;
                 auto& C = s->lc[l];
                 c[n] = s->lctx(l);
                 c[n].cimg = r == 0 ? &C.ovr : &C.plain;
                 a[n].Ain = C.A[cur[l]];
                 a[n].Aout = C.A[cur[l] ^ 1];
-                a[n].m = C.m[0];
+                a[n].m = C.m[mi];
                 if (r == 0) {
                     a[n].fcan = s->lv[l].d_f;
                     a[n].usq_old = last ? s->lv[l].d_usq_old : nullptr;
@@ -910,29 +919,30 @@ static int multi_transport32(mts_sim_s* s, int nsteps) {
                 n++;
             }
             const int mode = g == 0 && first ? 0 : 1;
-            if (g == 0 && pending) {  // the previous step's (2, 3) interface beside level 1's first substep
-                LaunchCtx c1 = s->lctx(1), c2 = s->lctx(2);
-                c1.cimg = &s->lc[1].ovr;
-                c2.cimg = &s->lc[2].ovr;
-                rc = launch_cinterface_beside(c1, c2, iface(1, 0), c[0], a[0], mode, mac & 1);
+            if (pending && g == 0) {
+                LaunchCtx c2 = s->lctx(1);
+                c2.cimg = &s->lc[1].ovr;
+                rc = launch_cinterface_step_fine(c[0], a[0], c2, f12);
+            } else if (pending && g == 1) {
+                LaunchCtx c2 = s->lctx(1), c3 = s->lctx(2);
+                c2.cimg = &s->lc[1].ovr;
+                c3.cimg = &s->lc[2].ovr;
+                rc = launch_cinterface_beside(c2, c3, f23, c[0], a[0], mode, mac & 1);
             } else {
                 rc = n == 1 ? launch_cmultistep(c[0], mode, mac & 1, a[0]) : launch_cmultistep3(n, c, a, mode, mac);
             }
             if (rc) return fail(rc, "operator combination has no kernel");
         }
-        if (Lmax > 1) {  // this step's (1, 2) interface
-            LaunchCtx c1 = s->lctx(0), c2 = s->lctx(1);
-            c1.cimg = &s->lc[0].ovr;
-            c2.cimg = &s->lc[1].ovr;
-            if ((rc = launch_cinterface(c1, c2, iface(0, 0)))) return fail(rc, "interface has no kernel");
-        }
+        mi ^= 1;
         if ((rc = s->launched())) return rc;
     }
-    if (Lmax == 3) {  // the last step's (2, 3) interface, as overrides for the export
-        LaunchCtx c1 = s->lctx(1), c2 = s->lctx(2);
-        c1.cimg = &s->lc[1].ovr;
-        c2.cimg = &s->lc[2].ovr;
-        if ((rc = launch_cinterface(c1, c2, iface(1, 0)))) return fail(rc, "interface has no kernel");
+    if (Lmax == 3) {  // the last step's interfaces, as overrides for the export
+        LaunchCtx c1 = s->lctx(0), c2 = s->lctx(1), c3 = s->lctx(2);
+        c1.cimg = &s->lc[0].ovr;
+        c2.cimg = &s->lc[1].ovr;
+        c3.cimg = &s->lc[2].ovr;
+        if ((rc = launch_cinterface(c1, c2, iface(0, mi ^ 1))) || (rc = launch_cinterface(c2, c3, iface(1, mi ^ 1))))
+            return fail(rc, "interface has no kernel");
     }
     if (Lmax == 2) {  // the last step's interface, as overrides for the export
         LaunchCtx c1 = s->lctx(0), c2 = s->lctx(1);

@@ -158,6 +158,10 @@ int launch_cinterface(const LaunchCtx& c1, const LaunchCtx& c2, const IfSpec& s)
 // operators), all with the same mode; bit i of mac: image i uses the step's
 // stored fields.
 int launch_cmultistep3(int n, const LaunchCtx* c, const MbSpec* s, int mode, int mac);
+// The interface of levels (1, 2) of a three-level lattice in the launch of
+// the next step's first level-1 substep: fine buffer values update their
+// cells there, coarse buffer values are written as overrides.
+int launch_cinterface_step_fine(const LaunchCtx& c1, const MbSpec& s1, const LaunchCtx& c2, const IfSpec& f);
 // An interface of a pair of levels written as overrides, and in the same
 // launch a substep of one level image that reads none of them.
 int launch_cinterface_beside(const LaunchCtx& c1, const LaunchCtx& c2, const IfSpec& f, const LaunchCtx& ca,
