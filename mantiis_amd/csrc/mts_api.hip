@@ -747,6 +747,14 @@ static int create_multi(mts_sim_s* s, int nx, int ny, const int32_t* pore, const
     if (rc) return rc;
     if ((rc = multi_levels_device(nx, ny, pore, torus, K, s->stream, lvl, err))) return rc;
     if ((rc = multi_cells(nx, ny, pore, kn, torus, lvl, s->mg, err))) return rc;
+    {  // a domain with no wall near some level (a wall-free torus) leaves the finer levels empty
+        std::vector<int64_t> count(s->mg.maxLevel + 1, 0);
+        for (int64_t c = 0; c < s->mg.ncell; c++) count[s->mg.level[c]]++;
+        for (int l = 1; l <= s->mg.maxLevel; l++)
+            if (!count[l])
+                return fail(MTS_ERR_UNSUPPORTED, "multi-block lattice whose level " + std::to_string(l) +
+                                                     " holds no cells (no wall near it); use multiblock = 0");
+    }
     if ((rc = multi_grid_taus(s->ph, s->mg, err))) return rc;
     if ((rc = build_level_images(s->mg, s->limg, s->mparams, s->ph, err, false))) return rc;
     if ((rc = multi_tables_device(s->mg, &s->limg, s->stream, err))) return rc;
@@ -1158,7 +1166,7 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
         int rc = create_multi(s, nx, ny, pore, kn, locpore, err);
         if (rc) {
             delete s;
-            return fail(rc, err);
+            return err.empty() ? rc : fail(rc, err);  // empty: the inner fail() holds the message
         }
         *out = s;
         return MTS_OK;
@@ -1166,7 +1174,7 @@ int mts_create(int nx, int ny, const int32_t* pore, const double* kn, const doub
     int rc = build_single_grid(nx, ny, pore, kn, locpore, s->ph, prm.periodic_y != 0, s->geo, err);
     if (rc) {
         delete s;
-        return fail(rc, err);
+        return err.empty() ? rc : fail(rc, err);
     }
     auto bail = [&](int code) {
         delete s;
@@ -1265,7 +1273,7 @@ int mts_grid_create(int nx, int ny, const int32_t* pore, const double* kn, const
     const int rc = build_grid(nx, ny, pore, kn, locpore, prm, ph, *g, err);
     if (rc) {
         delete g;
-        return fail(rc, err);
+        return err.empty() ? rc : fail(rc, err);
     }
     *out = g;
     return MTS_OK;
@@ -1289,7 +1297,7 @@ int mts_grid_arrays(mts_grid g, int64_t* gridID, int32_t* gridIJ, int32_t* level
     if (taus && g->multi && g->mg.taus.empty()) {
         std::string err;
         const int rc = multi_grid_taus(g->ph, g->mg, err);
-        if (rc) return fail(rc, err);
+        if (rc) return err.empty() ? rc : fail(rc, err);
     }
     const int64_t n = g->n();
     for (int64_t c = 0; c < n; c++) {

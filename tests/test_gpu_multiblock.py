@@ -366,3 +366,16 @@ def test_multiblock_fp32_mdbb_is_refused():
     with pytest.raises(MtsError) as e:
         Lattice(pore, kn, lp, precision="fp32dev", multiblock=3, wall="MDBB")
     assert "FP64" in str(e.value)
+
+
+@pytest.mark.parametrize("levels", [1, 3])
+def test_multiblock_wall_free_torus_is_refused(levels):
+    """A torus with no solid has no wall to refine at: every cell lands on
+    the coarsest level and the finer ones are empty.  The lattice refuses it
+    with a clear error (the single-block lattice covers such a domain)."""
+    from mantiis_amd.hip import MtsError
+    pore = np.ones((64, 64), np.int32)
+    kn, lp = G.uniform_kn(pore, 0.1, MFP_DX)
+    with pytest.raises(MtsError) as e:
+        Lattice(pore, kn, lp, precision="fp32dev", multiblock=levels, periodic_y=1)
+    assert "holds no cells" in str(e.value)
