@@ -138,7 +138,7 @@ struct mts_batch_s {
                 if (tiled) tq.insert(tq.end(), tiles_of[g].begin(), tiles_of[g].end());
             }
         nactive = (int)q.size();
-        q.resize((size_t)grid_for(nactive), BatBlock{0, 0, -1, -1});
+        q.resize((size_t)grid_for(nactive), BatBlock{0, 0, -1, -1, -1});
         if (!q.empty())
             BT_TRY(hipMemcpyAsync(d_queue, q.data(), q.size() * sizeof(BatBlock), hipMemcpyHostToDevice, stream));
         if (tiled) {
@@ -521,7 +521,8 @@ int mts_batch_create(int nb, const int* nx, const int* ny, const int32_t* const*
         }
         for (int64_t s0 = 0; s0 < n; s0 += 256) {
             const int64_t s1 = std::min<int64_t>(s0 + 256, n);
-            const BatBlock B{(int32_t)(base + s0), (int32_t)(base + s1), g, (int32_t)b->hblocks.size()};
+            const BatBlock B{(int32_t)(base + s0), (int32_t)(base + s1), g, (int32_t)b->hblocks.size(),
+                             geo.params.size() == 1 ? poff : -1};
             b->blocks_of[g].push_back(B.id);
             b->hblocks.push_back(B);
         }

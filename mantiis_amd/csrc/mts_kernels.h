@@ -169,10 +169,12 @@ int launch_cinterface_beside(const LaunchCtx& c1, const LaunchCtx& c2, const IfS
 // Batched single-block lattices (cell order): one block of 256 cells of one
 // lattice per queue entry.
 // A 256-cell block of the batched image: cells [start, end) of lattice
-// geom; id = its index among the batch's blocks.  An entry with start == end
-// pads the work queue to the grid size.
+// geom; id = its index among the batch's blocks; pair = the lattice's
+// parameter pair when it has only one (its cells then skip their pair
+// index), -1 otherwise.  An entry with start == end pads the work queue to
+// the grid size.
 struct BatBlock {
-    int32_t start, end, geom, id;
+    int32_t start, end, geom, id, pair;
 };
 struct BatchLaunch {
     LaunchCtx c;  // cimg: the concatenated image
