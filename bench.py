@@ -355,7 +355,12 @@ def bench_batched(args):
     cells = b.cells()
     ncell = int(cells.sum())
     print(f"[bench] batch of {ncell} cells set up in {t_setup:.1f} s", file=sys.stderr, flush=True)
-    thr, xrun, tab = b.tune(8)
+    if args.launch:  # a fixed launch shape (profiler runs: the tuner's timings are distorted under counters)
+        thr, xrun = (int(v) for v in args.launch.split(","))
+        b.set_config(thr, xrun)
+        tab = None
+    else:
+        thr, xrun, tab = b.tune(8)
     spl, units = b.kernel()  # steps per launch of the step kernel (2: the tile kernels)
     timer = StreamTimer(b.stream_handle())
     steady = None
@@ -441,7 +446,8 @@ def bench_batched(args):
         "autotune": {"threads_per_block": thr, "xcd_run": xrun,
                      "ms_per_step": [[round(v, 4) for v in row] for row in tab.tolist()],
                      "candidates": "rows: threads per 256-cell block 64, 128, 256; columns: XCD run 0, 8, 32; "
-                                   "8 steps each from the first state, state restored"},
+                                   "8 steps each from the first state, state restored"}
+                    if tab is not None else {"threads_per_block": thr, "xcd_run": xrun, "fixed": True},
         "batch_setup": {"generation_s": round(t_gen, 1), "setup_s": round(t_setup, 1)},
     }
     print(json.dumps(out), flush=True)
@@ -614,6 +620,8 @@ def main():
                          "the HIP runtime, below mts_batch_s::chunk_graph)")
     ap.add_argument("--no-steady", action="store_true", help="batched / quadtree: skip the run to steady state")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--launch", default="",
+                    help="batched: THREADS,XCD_RUN -- a fixed launch shape instead of the autotuner's choice")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU sample steps (default: quadtree 1000, shanchen 16)")
     ap.add_argument("--cpu-n", type=int, default=256, help="quadtree: lattice side of the CPU sample")
     ap.add_argument("--n", type=int, default=0, help="lattice side of the transport workload (default 1024)")
