@@ -331,7 +331,8 @@ int mts_batch_stream(mts_batch b, void** stream);
  * configuration of the batched step -- threads per 256-cell block {64, 128,
  * 256} (each thread then takes 4, 2 or 1 cells) x XCD run length of the
  * queue map {0 (queue order), 8, 32} -- for steps_each steps from the current
- * state, keeps the fastest and puts the state back (the configuration
+ * state, keeps the fastest (of the fastest width's queue maps within 1% of
+ * it, the longest XCD run) and puts the state back (the configuration
  * changes no result).  steps_each == 0 takes the default (256, 32) untimed;
  * steps_each < 0 only reports.  mts_batch_run tunes with 8 steps on its first
  * call if this was never called.  Outputs (may be null): the configuration

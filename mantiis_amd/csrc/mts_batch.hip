@@ -319,6 +319,14 @@ struct mts_batch_s {
             xrun = 32;
             return rc ? rc : rrc;
         }
+        // the short timings do not separate the queue maps of one launch width
+        // within about 1%: of those that close to the fastest, take the
+        // longest XCD run (the fewest bytes fetched from memory)
+        const int b0 = best;
+        for (int k = 0; k < NCAND; k++)
+            if (tune_ms[k] > 0.0 && cand_threads(k) == cand_threads(b0) && cand_xrun(k) > cand_xrun(best) &&
+                tune_ms[k] <= 1.01 * tune_ms[b0])
+                best = k;
         threads = cand_threads(best);
         xrun = cand_xrun(best);
         // a captured chunk holds the old configuration
